@@ -1,0 +1,145 @@
+"""ctypes binding of the C ABI declared in include/fdtd2d.h.
+
+The shared library is built in-tree (``python -m fdtd2d_amd.build``) and
+loaded from ``fdtd2d_amd/lib/libfdtd2d.so``. There is no fallback: if the
+library is missing, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfdtd2d.so"
+
+OK, EINVAL, ENOMEM, EDEVICE, EIO, ESTATE, EUNSUPPORTED = range(7)
+MEDIUM_TABLE, MEDIUM_DENSITY = 0, 1
+EZ, HX, HY, JZ = 0, 1, 2, 3
+PSI_HXY, PSI_HYX, PSI_EZX, PSI_EZY = 0, 1, 2, 3
+
+_f = C.POINTER(C.c_float)
+_i = C.POINTER(C.c_int32)
+
+
+class ProblemC(C.Structure):
+    """Mirror of ``struct fdtd2d_problem``."""
+
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("npml", C.c_int32),
+        ("nsteps", C.c_int32),
+        ("dt", C.c_double),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("db", C.c_float),
+        ("medium_kind", C.c_int32),
+        ("material", _i),
+        ("nmaterials", C.c_int32),
+        ("ca", _f),
+        ("cb", _f),
+        ("ne", _f),
+        ("pec_material", C.c_int32),
+        ("density", _f),
+        ("ca0", C.c_float),
+        ("cb0", C.c_float),
+        ("cj", C.c_float),
+        ("jz_decay", C.c_float),
+        ("jz_dt", C.c_double),
+        ("be_x", _f * 2),
+        ("ce_x", _f * 2),
+        ("bh_x", _f * 2),
+        ("ch_x", _f * 2),
+        ("be_y", _f * 2),
+        ("ce_y", _f * 2),
+        ("bh_y", _f * 2),
+        ("ch_y", _f * 2),
+        ("den_ex", _f),
+        ("den_hx", _f),
+        ("den_ey", _f),
+        ("den_hy", _f),
+        ("dx_f", C.c_float),
+        ("dy_f", C.c_float),
+        ("nsrc", C.c_int32),
+        ("src_i", _i),
+        ("src_j", _i),
+        ("src_table", _f),
+        ("ndet", C.c_int32),
+        ("det_i", _i),
+        ("det_j", _i),
+        ("ncomp", C.c_int32),
+        ("comp_i", _i),
+        ("comp_j", _i),
+    ]
+
+
+class FDTD2DError(RuntimeError):
+    def __init__(self, code: int, where: str, detail: str):
+        super().__init__(f"{where}: status {code}: {detail}")
+        self.code = code
+
+
+_lib = None
+
+# (name, restype, argtypes) for every function declared in include/fdtd2d.h
+SIGNATURES = [
+    ("fdtd2d_strerror", C.c_char_p, [C.c_int]),
+    ("fdtd2d_last_error", C.c_char_p, []),
+    ("fdtd2d_abi_version", C.c_int, []),
+    ("fdtd2d_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("fdtd2d_setup_from_ini", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_setup_ini_status", C.c_int, [C.c_void_p]),
+    ("fdtd2d_setup_workload", C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_setup_box", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_setup_problem", C.POINTER(ProblemC), [C.c_void_p]),
+    ("fdtd2d_setup_problem_mut", C.POINTER(ProblemC), [C.c_void_p]),
+    ("fdtd2d_setup_get", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    ("fdtd2d_setup_destroy", None, [C.c_void_p]),
+    ("fdtd2d_write_outputs", C.c_int, [C.c_void_p, C.c_char_p, _f, _f, C.c_double]),
+    ("fdtd2d_plan_create", C.c_int, [C.POINTER(ProblemC), C.c_int, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_plan_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("fdtd2d_plan_set_tblock", C.c_int, [C.c_void_p, C.c_int]),
+    ("fdtd2d_plan_get_info", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    ("fdtd2d_run", C.c_int, [C.c_void_p, C.c_int]),
+    ("fdtd2d_sync", C.c_int, [C.c_void_p]),
+    ("fdtd2d_steps_done", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("fdtd2d_read_detectors", C.c_int, [C.c_void_p, _f, C.c_size_t, _f, C.c_size_t]),
+    ("fdtd2d_read_field", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
+    ("fdtd2d_write_field", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
+    ("fdtd2d_read_psi", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
+    ("fdtd2d_write_psi", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
+    ("fdtd2d_plan_footprint", C.c_int, [C.POINTER(ProblemC), C.POINTER(C.c_uint64)]),
+    ("fdtd2d_plan_destroy", C.c_int, [C.c_void_p]),
+    ("fdtd2d_plan_time_kernels", C.c_int, [C.c_void_p, C.c_int]),
+    ("fdtd2d_plan_kernel_ms", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+]
+
+
+def lib() -> C.CDLL:
+    """Loads libfdtd2d.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python -m fdtd2d_amd.build` "
+                "(the HIP stepper has no CPU fallback)"
+            )
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    n = C.c_int()
+    rc = lib().fdtd2d_device_count(C.byref(n))
+    return n.value if rc == OK else 0
+
+
+def check(rc: int, where: str) -> None:
+    if rc != OK:
+        L = lib()
+        raise FDTD2DError(rc, where, f"{L.fdtd2d_strerror(rc).decode()} ({L.fdtd2d_last_error().decode()})")

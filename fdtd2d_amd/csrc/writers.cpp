@@ -1,0 +1,134 @@
+// writers.cpp - the reference's detector output files, written by the host
+// driver from the traces a plan returns (formats of F/kernel.cu:2409-2493,
+// 2653-2676 and F/FDTD_cuda.cu:3647-3650).
+#include <cstdio>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "errors.h"
+#include "setup_internal.h"
+
+namespace {
+
+constexpr double kDetMult = 1000;          // DET_MULT, F/Settings.cuh:72
+constexpr double kTimePrint = 1e12;        // TIME_PRINT_COEF, F/Settings.cuh:13
+
+FILE *open_in(const std::string &dir, const char *name, const char *mode) {
+  std::string path = dir.empty() ? std::string(name) : dir + "/" + name;
+  return std::fopen(path.c_str(), mode);
+}
+
+}  // namespace
+
+extern "C" int fdtd2d_write_outputs(const fdtd2d_setup *setup, const char *dir, const float *det,
+                                    const float *comp, double seconds) {
+  using fdtd2d::set_error;
+  if (!setup || !dir) return set_error(FDTD2D_EINVAL, "null argument");
+  const fdtd2d::Setup &S = setup->s;
+  const fdtd2d_problem &p = S.p;
+  const int nMax = p.nsteps;
+  const int ndet = p.ndet, ncomp = p.ncomp;
+  const double dt = p.dt;
+  const std::string d(dir);
+
+  // Detect.txt: t [ps], sum of Ez over the detector array x 1000, and a third
+  // column that is uninitialised in the reference (written 0 here).
+  if (det) {
+    FILE *f = open_in(d, "Detect.txt", "w");
+    if (!f) return set_error(FDTD2D_EIO, "cannot write Detect.txt in " + d);
+    for (int i = 0; i < nMax; ++i) {
+      double sum = 0, sum1 = 0;
+      for (int k = 0; k < ndet; ++k) sum += det[static_cast<size_t>(i) * ndet + k];
+      std::fprintf(f, "%lf\t%12lf\t%12lf\n ", (dt * i) * kTimePrint, sum * kDetMult, sum1 * kDetMult);
+    }
+    std::fclose(f);
+  }
+  if (comp) {
+    const size_t plane = static_cast<size_t>(nMax + 1) * ncomp;
+    // detAll.txt: t [us], Ex, Ey (0), Ez, Hx, Hy sums, Hz (0), all x 1000
+    FILE *f = open_in(d, "detAll.txt", "w");
+    if (!f) return set_error(FDTD2D_EIO, "cannot write detAll.txt in " + d);
+    for (int i = 0; i < nMax; ++i) {
+      double s3 = 0, s4 = 0, s5 = 0, z = 0;
+      for (int k = 0; k < ncomp; ++k) {
+        size_t q = static_cast<size_t>(i) * ncomp + k;
+        s3 += comp[q];
+        s4 += comp[plane + q];
+        s5 += comp[2 * plane + q];
+      }
+      std::fprintf(f, "%lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\n ", (dt * i) * 1000000, z * kDetMult,
+                   z * kDetMult, s3 * kDetMult, s4 * kDetMult, s5 * kDetMult, z * kDetMult);
+    }
+    std::fclose(f);
+    // detEz/detHx/detHy.txt: one value per element per line, blank line per step;
+    // detP.txt receives only the line breaks.
+    const char *names[3] = {"detEz.txt", "detHx.txt", "detHy.txt"};
+    FILE *fs[4];
+    for (int q = 0; q < 3; ++q) {
+      fs[q] = open_in(d, names[q], "w");
+      if (!fs[q]) return set_error(FDTD2D_EIO, std::string("cannot write ") + names[q]);
+    }
+    fs[3] = open_in(d, "detP.txt", "w");
+    if (!fs[3]) return set_error(FDTD2D_EIO, "cannot write detP.txt");
+    const int rows = ncomp / (S.det_cols > 0 ? S.det_cols : 1);
+    const int cols = S.det_cols > 0 ? S.det_cols : 1;
+    for (int i = 0; i < nMax; ++i) {
+      for (int j = 0; j < rows; ++j) {
+        for (int k = 0; k < cols; ++k)
+          for (int q = 0; q < 3; ++q)
+            std::fprintf(fs[q], "%12lf\t", comp[q * plane + static_cast<size_t>(i) * ncomp + j * cols + k] * kDetMult);
+        for (int q = 0; q < 4; ++q) std::fprintf(fs[q], "\n");
+      }
+      for (int q = 0; q < 4; ++q) std::fprintf(fs[q], "\n");
+    }
+    for (int q = 0; q < 4; ++q) std::fclose(fs[q]);
+  }
+  // SourceP.txt: the source envelope, one "%f" line per step index 0..nMax-1.
+  {
+    FILE *f = open_in(d, "SourceP.txt", "w");
+    if (!f) return set_error(FDTD2D_EIO, "cannot write SourceP.txt");
+    for (int k = 0; k < nMax; ++k)
+      std::fprintf(f, "%f\n", k < static_cast<int>(S.source_profile.size()) ? S.source_profile[k] : 0.f);
+    std::fclose(f);
+  }
+  // log.txt (append): grid, time step and wall time.
+  {
+    FILE *f = open_in(d, "log.txt", "a");
+    if (!f) return set_error(FDTD2D_EIO, "cannot append log.txt");
+    char tdate[64], ttime[64];
+    std::time_t now = std::time(nullptr);
+    std::strftime(tdate, sizeof tdate, "%m/%d/%y", std::localtime(&now));
+    std::strftime(ttime, sizeof ttime, "%H:%M:%S", std::localtime(&now));
+    const int Imax = p.nx, Jmax = p.ny, Kmax = 1;
+    std::fprintf(f, "-------------------------------------------------------------------------\n");
+    std::fprintf(f, "OS date:\t\t\t\t%s\n", tdate);
+    std::fprintf(f, "OS time:\t\t\t\t%s\n", ttime);
+    std::fprintf(f, "-------------------------------------------------------------------------\n");
+    std::fprintf(f,
+                 "NET PARAMETERS \n Ex(%d, %d, %d) \n Ey(%d, %d, %d) \n Ez(%d, %d, %d) \n Hx(%d, %d, %d) \n "
+                 "Hy(%d, %d, %d) \n Hz(%d, %d, %d)\n\n(dx ,dy, dz) - dt: (%f mm, %f mm, %f mm) - %f ns\n",
+                 Imax - 1, Jmax, Kmax - 1, Imax, Jmax - 1, Kmax - 1, Imax, Jmax, Kmax, Imax, Jmax - 1, Kmax,
+                 Imax - 1, Jmax, Kmax, Imax - 1, Jmax - 1, Kmax - 1, p.dx * 1000, p.dy * 1000, p.dx * 1000,
+                 dt * 1e9);
+    std::fprintf(f, "\nSIMULATION PARAMETERS\n Total Simulation time: %d time steps or %e seconds\n", nMax,
+                 nMax * dt);
+    auto get = [&](const char *k) {
+      auto it = S.params.find(k);
+      return it == S.params.end() ? 0.0 : it->second;
+    };
+    std::fprintf(f,
+                 "\nSAVE DATA PARAMETER\nSlises saved: %d\nData saved from time %d to time %d \nTime sieve "
+                 "parameter: %d \nNet sieve parameter: %d\nData multiplier: %f\n",
+                 0, static_cast<int>(get("From")), static_cast<int>(get("To")), static_cast<int>(get("TimeSieve")),
+                 static_cast<int>(get("GridSieve")), 1.0);
+    std::fprintf(f,
+                 "\nSORCE AND DETECTOR\n Source point: (%d, %d, %d)\n Detector point(%d, %d, %d)\n Detector "
+                 "size: %dx%d points",
+                 static_cast<int>(get("Isource")), static_cast<int>(get("Jsource")), 1,
+                 static_cast<int>(get("Idetect")), static_cast<int>(get("Jdetect")), 1, 10, 10);
+    std::fprintf(f, "\n\nCalculation on GPU\n Calculation time: %f seconds\n\n", seconds);
+    std::fclose(f);
+  }
+  return FDTD2D_OK;
+}

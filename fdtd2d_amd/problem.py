@@ -1,0 +1,342 @@
+"""Problems, setups and plans: the Python face of include/fdtd2d.h.
+
+``Problem`` owns numpy arrays and produces the C struct; ``Setup`` wraps the
+native builders (Settings.ini, synthetic workloads, test boxes); ``Plan`` is a
+device plan (the replacement of the reference's DevFieldCalculation,
+F/kernel.cu:1928).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _fp(a: Optional[np.ndarray]):
+    return a.ctypes.data_as(L._f) if a is not None and a.size else L._f()
+
+
+def _ip(a: Optional[np.ndarray]):
+    return a.ctypes.data_as(L._i) if a is not None and a.size else L._i()
+
+
+def _copy_f(ptr, n: int) -> np.ndarray:
+    if n <= 0 or not ptr:
+        return np.zeros(max(n, 0), np.float32)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.float32, copy=True)
+
+
+def _copy_i(ptr, n: int) -> np.ndarray:
+    if n <= 0 or not ptr:
+        return np.zeros(max(n, 0), np.int32)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.int32, copy=True)
+
+
+@dataclass
+class Problem:
+    """Plain-numpy mirror of ``fdtd2d_problem`` (field meanings as there)."""
+
+    nx: int
+    ny: int
+    npml: int
+    nsteps: int
+    dt: float
+    dx: float
+    dy: float
+    db: float
+    medium_kind: int = L.MEDIUM_DENSITY
+    material: Optional[np.ndarray] = None  # int32 [ny, nx]
+    ca: Optional[np.ndarray] = None
+    cb: Optional[np.ndarray] = None
+    ne: Optional[np.ndarray] = None
+    pec_material: int = -1
+    density: Optional[np.ndarray] = None  # float32 [ny, nx]
+    ca0: float = 1.0
+    cb0: float = 0.0
+    cj: float = 2.81794e-8
+    jz_decay: float = 1.0
+    jz_dt: float = 0.0
+    be_x: list = field(default_factory=lambda: [None, None])
+    ce_x: list = field(default_factory=lambda: [None, None])
+    bh_x: list = field(default_factory=lambda: [None, None])
+    ch_x: list = field(default_factory=lambda: [None, None])
+    be_y: list = field(default_factory=lambda: [None, None])
+    ce_y: list = field(default_factory=lambda: [None, None])
+    bh_y: list = field(default_factory=lambda: [None, None])
+    ch_y: list = field(default_factory=lambda: [None, None])
+    den_ex: Optional[np.ndarray] = None
+    den_hx: Optional[np.ndarray] = None
+    den_ey: Optional[np.ndarray] = None
+    den_hy: Optional[np.ndarray] = None
+    dx_f: float = 0.0
+    dy_f: float = 0.0
+    src_i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    src_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    src_table: np.ndarray = field(default_factory=lambda: np.zeros((1, 0), np.float32))
+    det_i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    det_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    comp_i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    comp_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+    # -------------------------------------------------------------- C struct
+    def to_c(self):
+        """Returns (struct, keepalive); keep both alive while the struct is used."""
+        keep = []
+
+        def f32(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            keep.append(a)
+            return a
+
+        def i32(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            keep.append(a)
+            return a
+
+        s = L.ProblemC()
+        s.nx, s.ny, s.npml, s.nsteps = self.nx, self.ny, self.npml, self.nsteps
+        s.dt, s.dx, s.dy = self.dt, self.dx, self.dy
+        s.db = self.db
+        s.medium_kind = self.medium_kind
+        s.material = _ip(i32(self.material))
+        s.nmaterials = 0 if self.ca is None else int(np.asarray(self.ca).size)
+        s.ca, s.cb, s.ne = _fp(f32(self.ca)), _fp(f32(self.cb)), _fp(f32(self.ne))
+        s.pec_material = self.pec_material
+        s.density = _fp(f32(self.density))
+        s.ca0, s.cb0, s.cj, s.jz_decay, s.jz_dt = self.ca0, self.cb0, self.cj, self.jz_decay, self.jz_dt
+        for name in ("be_x", "ce_x", "bh_x", "ch_x", "be_y", "ce_y", "bh_y", "ch_y"):
+            arr = getattr(s, name)
+            for side in range(2):
+                arr[side] = _fp(f32(getattr(self, name)[side]))
+        s.den_ex, s.den_hx = _fp(f32(self.den_ex)), _fp(f32(self.den_hx))
+        s.den_ey, s.den_hy = _fp(f32(self.den_ey)), _fp(f32(self.den_hy))
+        s.dx_f, s.dy_f = self.dx_f, self.dy_f
+        s.nsrc = int(len(self.src_i))
+        s.src_i, s.src_j = _ip(i32(self.src_i)), _ip(i32(self.src_j))
+        s.src_table = _fp(f32(self.src_table))
+        s.ndet = int(len(self.det_i))
+        s.det_i, s.det_j = _ip(i32(self.det_i)), _ip(i32(self.det_j))
+        s.ncomp = int(len(self.comp_i))
+        s.comp_i, s.comp_j = _ip(i32(self.comp_i)), _ip(i32(self.comp_j))
+        return s, keep
+
+    @classmethod
+    def from_c(cls, s: L.ProblemC) -> "Problem":
+        nx, ny, P, ns = s.nx, s.ny, s.npml, s.nsteps
+        nm = s.nmaterials
+        table = s.medium_kind == L.MEDIUM_TABLE
+        p = cls(
+            nx=nx, ny=ny, npml=P, nsteps=ns, dt=s.dt, dx=s.dx, dy=s.dy, db=s.db,
+            medium_kind=s.medium_kind,
+            material=_copy_i(s.material, nx * ny).reshape(ny, nx) if table else None,
+            ca=_copy_f(s.ca, nm) if table else None,
+            cb=_copy_f(s.cb, nm) if table else None,
+            ne=_copy_f(s.ne, nm) if table else None,
+            pec_material=s.pec_material,
+            density=None if table else _copy_f(s.density, nx * ny).reshape(ny, nx),
+            ca0=s.ca0, cb0=s.cb0, cj=s.cj, jz_decay=s.jz_decay, jz_dt=s.jz_dt,
+            den_ex=_copy_f(s.den_ex, nx - 1), den_hx=_copy_f(s.den_hx, nx - 1),
+            den_ey=_copy_f(s.den_ey, ny - 1), den_hy=_copy_f(s.den_hy, ny - 1),
+            dx_f=s.dx_f, dy_f=s.dy_f,
+            src_i=_copy_i(s.src_i, s.nsrc), src_j=_copy_i(s.src_j, s.nsrc),
+            src_table=_copy_f(s.src_table, (ns + 1) * s.nsrc).reshape(ns + 1, s.nsrc),
+            det_i=_copy_i(s.det_i, s.ndet), det_j=_copy_i(s.det_j, s.ndet),
+            comp_i=_copy_i(s.comp_i, s.ncomp), comp_j=_copy_i(s.comp_j, s.ncomp),
+        )
+        for name, n in (("be_x", P), ("ce_x", P), ("bh_x", P - 1), ("ch_x", P - 1),
+                        ("be_y", P), ("ce_y", P), ("bh_y", P - 1), ("ch_y", P - 1)):
+            arr = getattr(s, name)
+            setattr(p, name, [_copy_f(arr[0], n) if P > 0 else None, _copy_f(arr[1], n) if P > 0 else None])
+        return p
+
+    def footprint(self) -> int:
+        s, keep = self.to_c()
+        out = C.c_uint64()
+        L.check(L.lib().fdtd2d_plan_footprint(C.byref(s), C.byref(out)), "fdtd2d_plan_footprint")
+        return int(out.value)
+
+    def cell_updates(self, steps: Optional[int] = None) -> int:
+        return self.nx * self.ny * (self.nsteps if steps is None else steps)
+
+
+class Setup:
+    """Owner of a native problem built from Settings.ini, a workload or a box."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_ini(cls, path: str) -> "Setup":
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_setup_from_ini(str(path).encode(), C.byref(h)), "fdtd2d_setup_from_ini")
+        return cls(h.value)
+
+    @classmethod
+    def workload(cls, workload: int, seed: int = 0, shot: int = 0, steps: int = 0) -> "Setup":
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_setup_workload(workload, seed, shot, steps, C.byref(h)), "fdtd2d_setup_workload")
+        return cls(h.value)
+
+    @classmethod
+    def box(cls, nx: int, ny: int, npml: int, nsteps: int, ds: float = 1e-3) -> "Setup":
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_setup_box(nx, ny, npml, nsteps, ds, C.byref(h)), "fdtd2d_setup_box")
+        return cls(h.value)
+
+    @property
+    def c_problem(self):
+        return L.lib().fdtd2d_setup_problem(self._h)
+
+    @property
+    def ini_status(self) -> int:
+        return L.lib().fdtd2d_setup_ini_status(self._h)
+
+    def problem(self) -> Problem:
+        return Problem.from_c(self.c_problem.contents)
+
+    def get(self, name: str) -> float:
+        v = C.c_double()
+        L.check(L.lib().fdtd2d_setup_get(self._h, name.encode(), C.byref(v)), f"fdtd2d_setup_get({name})")
+        return v.value
+
+    def write_outputs(self, out_dir: str, det: np.ndarray, comp: np.ndarray, seconds: float = 0.0) -> None:
+        d = np.ascontiguousarray(det, np.float32)
+        c = np.ascontiguousarray(comp, np.float32)
+        L.check(L.lib().fdtd2d_write_outputs(self._h, str(out_dir).encode(), _fp(d), _fp(c), seconds),
+                "fdtd2d_write_outputs")
+
+    def close(self) -> None:
+        if self._h:
+            L.lib().fdtd2d_setup_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+FIELD_SHAPES = {
+    L.EZ: lambda nx, ny: (ny, nx),
+    L.HX: lambda nx, ny: (ny - 1, nx),
+    L.HY: lambda nx, ny: (ny, nx - 1),
+    L.JZ: lambda nx, ny: (ny, nx),
+}
+
+
+def psi_shape(which: int, nx: int, ny: int, P: int):
+    if which == L.PSI_HXY:
+        return (2, P - 1, nx)
+    if which == L.PSI_EZY:
+        return (2, P, nx)
+    if which == L.PSI_HYX:
+        return (2, ny, P - 1)
+    return (2, ny, P)
+
+
+class Plan:
+    """A problem resident on one GPU (fdtd2d_plan_*)."""
+
+    def __init__(self, problem, device: int = 0, tblock: int = 0):
+        if isinstance(problem, Setup):
+            self._setup = problem
+            cp = problem.c_problem
+            self._keep = None
+            ptr = cp
+            s = cp.contents
+        else:
+            s, self._keep = problem.to_c()
+            ptr = C.byref(s)
+            self._setup = None
+        self.nx, self.ny, self.npml, self.nsteps = s.nx, s.ny, s.npml, s.nsteps
+        self.ndet, self.ncomp = s.ndet, s.ncomp
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_plan_create(ptr, device, C.byref(h)), "fdtd2d_plan_create")
+        self._h = h
+        if tblock:
+            self.set_tblock(tblock)
+
+    def set_tblock(self, t: int) -> None:
+        L.check(L.lib().fdtd2d_plan_set_tblock(self._h, t), "fdtd2d_plan_set_tblock")
+
+    def info(self, name: str) -> float:
+        v = C.c_double()
+        L.check(L.lib().fdtd2d_plan_get_info(self._h, name.encode(), C.byref(v)), "fdtd2d_plan_get_info")
+        return v.value
+
+    def set_stream(self, stream_ptr: int) -> None:
+        L.check(L.lib().fdtd2d_plan_set_stream(self._h, C.c_void_p(stream_ptr)), "fdtd2d_plan_set_stream")
+
+    def run(self, nsteps: int) -> None:
+        L.check(L.lib().fdtd2d_run(self._h, nsteps), "fdtd2d_run")
+
+    def sync(self) -> None:
+        L.check(L.lib().fdtd2d_sync(self._h), "fdtd2d_sync")
+
+    @property
+    def steps_done(self) -> int:
+        n = C.c_int()
+        L.check(L.lib().fdtd2d_steps_done(self._h, C.byref(n)), "fdtd2d_steps_done")
+        return n.value
+
+    def read_detectors(self):
+        det = np.zeros((self.nsteps + 1, self.ndet), np.float32)
+        comp = np.zeros((3, self.nsteps + 1, self.ncomp), np.float32)
+        L.check(L.lib().fdtd2d_read_detectors(self._h, _fp(det), det.size, _fp(comp), comp.size),
+                "fdtd2d_read_detectors")
+        return det, comp
+
+    def read_field(self, which: int) -> np.ndarray:
+        out = np.zeros(FIELD_SHAPES[which](self.nx, self.ny), np.float32)
+        L.check(L.lib().fdtd2d_read_field(self._h, which, _fp(out), out.size), "fdtd2d_read_field")
+        return out
+
+    def write_field(self, which: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, np.float32)
+        if v.shape != FIELD_SHAPES[which](self.nx, self.ny):
+            raise ValueError("field shape mismatch")
+        L.check(L.lib().fdtd2d_write_field(self._h, which, _fp(v), v.size), "fdtd2d_write_field")
+
+    def read_psi(self, which: int) -> np.ndarray:
+        out = np.zeros(psi_shape(which, self.nx, self.ny, self.npml), np.float32)
+        L.check(L.lib().fdtd2d_read_psi(self._h, which, _fp(out), out.size), "fdtd2d_read_psi")
+        return out
+
+    def write_psi(self, which: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, np.float32)
+        L.check(L.lib().fdtd2d_write_psi(self._h, which, _fp(v), v.size), "fdtd2d_write_psi")
+
+    def time_kernels(self, enable: bool = True) -> None:
+        L.check(L.lib().fdtd2d_plan_time_kernels(self._h, 1 if enable else 0), "fdtd2d_plan_time_kernels")
+
+    def kernel_ms(self):
+        ms, n = C.c_double(), C.c_int()
+        L.check(L.lib().fdtd2d_plan_kernel_ms(self._h, C.byref(ms), C.byref(n)), "fdtd2d_plan_kernel_ms")
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().fdtd2d_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,198 @@
+/*
+ * fdtd2d.h - C ABI of the MI355X (gfx950) 2D TMz FDTD time-stepper.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no HIP or torch
+ * types. Every call returns an int status (FDTD2D_OK == 0); the text of the
+ * last failure on the calling thread is available from fdtd2d_last_error().
+ *
+ * Reference interface each entry point replaces (Virtuoso2598/FDTD_2D,
+ * FDTD_2D_Full/ abbreviated F/):
+ *   the whole device side of one run is DevFieldCalculation(void)
+ *   (F/Kernel.cuh:102, F/kernel.cu:1928), which reads host globals
+ *   (F/Ext_kernel.cuh:8-258) and writes ~22 text files. Here the globals
+ *   become an fdtd2d_problem, device state lives in an opaque plan, and file
+ *   writing stays in the host driver (fdtd2d_write_outputs).
+ *
+ * Field layout (the reference's own index rules, F/kernel.cu:127-537):
+ *   Ez, Jz : ny rows x nx columns, element (j,i) at j*nx+i
+ *   Hx     : (ny-1) x nx        (pitch nx)
+ *   Hy     : ny x (nx-1)        (pitch nx-1)
+ * nx == Imax, ny == Jmax. The device pads every field to a common pitch; the
+ * read/write calls below convert to and from these reference layouts.
+ */
+#ifndef FDTD2D_H
+#define FDTD2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD2D_ABI_VERSION 1
+
+/* status codes */
+enum {
+  FDTD2D_OK = 0,
+  FDTD2D_EINVAL = 1,      /* bad argument or inconsistent problem */
+  FDTD2D_ENOMEM = 2,      /* host or device allocation failed / over budget */
+  FDTD2D_EDEVICE = 3,     /* HIP runtime error (launch, copy, fault at sync) */
+  FDTD2D_EIO = 4,         /* file could not be read or written */
+  FDTD2D_ESTATE = 5,      /* call not valid in the object's current state */
+  FDTD2D_EUNSUPPORTED = 6 /* valid request this build does not implement */
+};
+
+/* medium_kind */
+enum {
+  FDTD2D_MEDIUM_TABLE = 0,  /* int32 material id per cell + CA/CB/Ne tables (reference ID3) */
+  FDTD2D_MEDIUM_DENSITY = 1 /* float electron density per cell, uniform CA/CB */
+};
+
+/* fields for fdtd2d_read_field / fdtd2d_write_field */
+enum { FDTD2D_EZ = 0, FDTD2D_HX = 1, FDTD2D_HY = 2, FDTD2D_JZ = 3 };
+
+/* CPML strip arrays for fdtd2d_read_psi / fdtd2d_write_psi */
+enum { FDTD2D_PSI_HXY = 0, FDTD2D_PSI_HYX = 1, FDTD2D_PSI_EZX = 2, FDTD2D_PSI_EZY = 3 };
+
+/*
+ * One simulation. Caller-owned; fdtd2d_plan_create copies everything it needs.
+ *
+ * Step n (1..nsteps) performs, in this order (F/kernel.cu:2073-2157):
+ *   Hx, then its y-CPML psi; Hy, then its x-CPML psi; Jz from E^n;
+ *   Ez (+x-CPML psi, then +y-CPML psi); soft sources Ez -= src_table[n][e];
+ *   detector and component sampling into row n.
+ */
+typedef struct fdtd2d_problem {
+  int32_t nx, ny;   /* Imax, Jmax (cells); both >= 2*npml + 4 */
+  int32_t npml;     /* CPML thickness P (reference PML_SIZE = 11); 0 = bare PEC box */
+  int32_t nsteps;   /* nMax: rows 0..nsteps of src_table and of the traces */
+  double dt, dx, dy;
+
+  /* H update: Hx += db*((Ez[j]-Ez[j+1])*den_hy[j]); reference DA == 1, DB == dt/mu0
+     for every material (F/FDTD_cuda.cu:521-533, 1196-1199). */
+  float db;
+
+  /* E update and plasma current. */
+  int32_t medium_kind;
+  const int32_t *material; /* TABLE: [ny*nx] id, reference ID3 */
+  int32_t nmaterials;
+  const float *ca, *cb, *ne; /* TABLE: [nmaterials] CA, CB and electron density (pne) */
+  int32_t pec_material;      /* TABLE: id whose Ez is forced to 0 (reference: 1); -1 = none */
+  const float *density;      /* DENSITY: [ny*nx] electron density, m^-3 */
+  float ca0, cb0;            /* DENSITY: CA, CB of every cell */
+  float cj;                  /* e^2/m_e as float, reference D_CJ = 2.81794e-8 */
+  float jz_decay;            /* Jz <- jz_decay*Jz + ((cj*Ez)*Ne)*jz_dt where Ne > 0 */
+  double jz_dt;              /* collisionless reference: jz_decay = 1, jz_dt = dt */
+
+  /* CPML. Index 0 = low side, 1 = high side; arrays are indexed by the distance
+     from the outer wall exactly as the reference's be_x_1/be_x_2 etc. */
+  const float *be_x[2], *ce_x[2]; /* [npml]   (index 0 unused by the updates) */
+  const float *bh_x[2], *ch_x[2]; /* [npml-1] */
+  const float *be_y[2], *ce_y[2]; /* [npml]   */
+  const float *bh_y[2], *ch_y[2]; /* [npml-1] */
+  const float *den_ex, *den_hx;   /* [nx-1] */
+  const float *den_ey, *den_hy;   /* [ny-1] */
+  float dx_f, dy_f;               /* psi divisors, as the float kernel argument */
+
+  /* Soft sources (reference AESA source, F/kernel.cu:621-659). */
+  int32_t nsrc;
+  const int32_t *src_i, *src_j;  /* cell of element e */
+  const float *src_table;        /* [(nsteps+1)*nsrc] value subtracted at step n */
+
+  /* Ez detectors (F/kernel.cu:762-805): row n of the trace <- Ez after step n. */
+  int32_t ndet;
+  const int32_t *det_i, *det_j;
+
+  /* Component detectors (F/kernel.cu:719-759): Ez, Hx, Hy at a cell. */
+  int32_t ncomp;
+  const int32_t *comp_i, *comp_j;
+} fdtd2d_problem;
+
+typedef struct fdtd2d_plan fdtd2d_plan;
+typedef struct fdtd2d_setup fdtd2d_setup;
+
+/* ------------------------------------------------------------------ errors */
+const char *fdtd2d_strerror(int status);
+const char *fdtd2d_last_error(void);
+int fdtd2d_abi_version(void);
+/* Number of visible HIP devices (0 when none; no device is initialised beyond
+   the runtime's own discovery). */
+int fdtd2d_device_count(int *count);
+
+/* ------------------------------------------------------------------ setup
+ * Host-side builders that own the arrays an fdtd2d_problem points to. */
+
+/* Settings.ini -> problem, with the reference's semantics:
+ * readINIparam + recuclParametersToMetric + setUp + buildObject +
+ * buildSourceProfile + initializeCPML (F/FDTD_cuda.cu:27-176, 305-382,
+ * 1122-1579, 2602-2659, 3380-3655). A missing or unparsable file falls back
+ * to the compile-time defaults, as the reference does (F/FDTD_cuda.cu:30-33);
+ * fdtd2d_setup_ini_status reports what happened (0 ok, -1 open failed, >0 first bad line). */
+int fdtd2d_setup_from_ini(const char *ini_path, fdtd2d_setup **out);
+int fdtd2d_setup_ini_status(const fdtd2d_setup *s);
+
+/* Synthetic workloads, seeded (std::mt19937_64):
+ *   1  400x400 plain method (one point source, linear plasma ramp, 4 detectors)
+ *   2  8192x8192 single grid (16-element steered array, parabolic plasma x (1+0.02 g))
+ *   3  one shot of the 256-shot uneven batch mix (shot = 0..255)
+ *   4  one frequency of a served sweep request (shot = request*64 + frequency index)
+ *   5  one shot of the mixed-size serving stream (shot = request index)
+ * steps_override > 0 replaces the workload's step count. */
+int fdtd2d_setup_workload(int workload, uint64_t seed, int shot, int steps_override,
+                          fdtd2d_setup **out);
+
+/* Tiny grids for tests: PEC box (npml = 0) or CPML box, uniform vacuum,
+ * no sources, no detectors. */
+int fdtd2d_setup_box(int nx, int ny, int npml, int nsteps, double ds_m, fdtd2d_setup **out);
+
+const fdtd2d_problem *fdtd2d_setup_problem(const fdtd2d_setup *s);
+fdtd2d_problem *fdtd2d_setup_problem_mut(fdtd2d_setup *s);
+/* Scalar value of a named setup parameter (e.g. "dt", "Imax", "nMax",
+ * "fCoef", "qantI"); returns FDTD2D_EINVAL for unknown names. */
+int fdtd2d_setup_get(const fdtd2d_setup *s, const char *name, double *value);
+void fdtd2d_setup_destroy(fdtd2d_setup *s);
+
+/* Reference output files (Detect.txt, detAll.txt, detEz/Hx/Hy.txt,
+ * SourceP.txt, log.txt ...; F/kernel.cu:2409-2493, F/FDTD_cuda.cu:3647-3650)
+ * written into dir from traces laid out as fdtd2d_read_detectors returns them. */
+int fdtd2d_write_outputs(const fdtd2d_setup *s, const char *dir, const float *det,
+                         const float *comp, double seconds);
+
+/* ------------------------------------------------------------------ plans
+ * Replaces DevMemoryAllocation / the DevFieldCalculation loop /
+ * DevMemoryFree (F/kernel.cu:1087-1618, 2065-2309, 2820-2987). */
+
+/* Copies the problem to device `device`; all state starts at zero. */
+int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out);
+/* Work on `stream` (a hipStream_t; NULL = the plan's own stream). */
+int fdtd2d_plan_set_stream(fdtd2d_plan *plan, void *stream);
+/* Time-blocking factor per launch (1..8; 0 = autotuned default). */
+int fdtd2d_plan_set_tblock(fdtd2d_plan *plan, int steps_per_launch);
+int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
+/* Advance nsteps steps (asynchronous on the plan's stream). */
+int fdtd2d_run(fdtd2d_plan *plan, int nsteps);
+int fdtd2d_sync(fdtd2d_plan *plan);
+int fdtd2d_steps_done(const fdtd2d_plan *plan, int *n);
+/* Traces: det [(nsteps+1)*ndet], comp [3][(nsteps+1)*ncomp] as Ez, Hx, Hy. */
+int fdtd2d_read_detectors(fdtd2d_plan *plan, float *det, size_t det_count, float *comp,
+                          size_t comp_count);
+int fdtd2d_read_field(fdtd2d_plan *plan, int which, float *out, size_t count);
+int fdtd2d_write_field(fdtd2d_plan *plan, int which, const float *in, size_t count);
+/* psi strips, reference layouts (F/kernel.cu:1172-1214): HXY [2][npml-1][nx],
+ * HYX [2][ny][npml-1], EZX [2][ny][npml], EZY [2][npml][nx]. */
+int fdtd2d_read_psi(fdtd2d_plan *plan, int which, float *out, size_t count);
+int fdtd2d_write_psi(fdtd2d_plan *plan, int which, const float *in, size_t count);
+/* Device bytes this plan holds (the footprint admission control counts). */
+int fdtd2d_plan_footprint(const fdtd2d_problem *p, uint64_t *bytes);
+int fdtd2d_plan_destroy(fdtd2d_plan *plan);
+
+/* Average duration (ms) of the last `count` step-kernel launches, timed with
+ * HIP events on the plan's stream; 0 disables timing. */
+int fdtd2d_plan_time_kernels(fdtd2d_plan *plan, int enable);
+int fdtd2d_plan_kernel_ms(fdtd2d_plan *plan, double *avg_ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD2D_H */
