@@ -99,6 +99,7 @@ SIGNATURES = [
     ("fdtd2d_plan_create", C.c_int, [C.POINTER(ProblemC), C.c_int, C.POINTER(C.c_void_p)]),
     ("fdtd2d_plan_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
     ("fdtd2d_plan_set_tblock", C.c_int, [C.c_void_p, C.c_int]),
+    ("fdtd2d_plan_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     ("fdtd2d_plan_get_info", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     ("fdtd2d_run", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_sync", C.c_int, [C.c_void_p]),

@@ -30,6 +30,10 @@ using fdtd2d::set_error;
 namespace {
 
 constexpr size_t kAlign = 256;
+// Scratch rows below each field plane: interior tiles direct the stores of
+// lanes/rows they do not own there, so their loop issues the same memory
+// operations every row (nothing reads these rows).
+constexpr int kTrashRows = 64;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -41,7 +45,7 @@ struct Layout {
   size_t off_med, off_lut;
   size_t off_col[8];  // den_ex, den_hx, hyx_b, hyx_c, ezx_b, ezx_c, hyx_slot, ezx_slot
   size_t off_row[8];  // den_ey, den_hy, hxy_b, hxy_c, ezy_b, ezy_c, hxy_slot, ezy_slot
-  size_t off_evoff, off_events, off_src, off_det, off_comp;
+  size_t off_rowinfo, off_evoff, off_events, off_src, off_det, off_comp;
   size_t n_events;
   size_t total;
 };
@@ -83,8 +87,9 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
     return r;
   };
   const size_t plane = static_cast<size_t>(L.ny) * L.pitch * 4;
+  const size_t fplane = static_cast<size_t>(L.ny + kTrashRows) * L.pitch * 4;
   for (int s = 0; s < 2; ++s)
-    for (int f = 0; f < 4; ++f) L.off_fields[s][f] = take(plane);
+    for (int f = 0; f < 4; ++f) L.off_fields[s][f] = take(fplane);
   const size_t ystrip = static_cast<size_t>(2 * std::max(L.P, 1)) * L.pitch * 4;
   const size_t xstrip = static_cast<size_t>(L.ny) * L.xs * 4;
   for (int s = 0; s < 2; ++s)
@@ -95,6 +100,7 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
   L.off_lut = take(256 * 16);
   for (int f = 0; f < 8; ++f) L.off_col[f] = take(static_cast<size_t>(L.pitch) * 4);
   for (int f = 0; f < 8; ++f) L.off_row[f] = take(static_cast<size_t>(L.ny) * 4);
+  L.off_rowinfo = take(static_cast<size_t>(L.ny) * 16);
   L.off_evoff = take(static_cast<size_t>(L.ny + 1) * 4);
   L.off_events = take(n_events * sizeof(fdtd2d::Event));
   L.off_src = take(static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4);
@@ -118,6 +124,12 @@ struct fdtd2d_plan {
   int nsteps = 0;
   int T = 0;  // 0: default
   int C = 1;
+  int band = 0;  // 0: automatic
+  int D = 1;     // rows of prefetch per wave (interior kernel)
+  int band_cpml = 0;               // rows per CPML-class work-group (0: automatic)
+  int4 *wg_buf = nullptr;          // device copy of the two work-group lists
+  hipStream_t aux = nullptr;       // CPML-class launches run here, beside the interior ones
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int ndet = 0, ncomp = 0, nsrc = 0;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -141,8 +153,7 @@ int hip_fail(hipError_t e, const char *what) {
 
 // Band height: enough bands for >= ~8 waves per SIMD on 256 CUs, at least 32
 // rows, at most 256.
-int pick_band(int nx, int ny, int T) {
-  const int VW = 64 - 2 * T;
+int pick_band(int nx, int ny, int VW, int T) {
   const int ntx = (nx + VW - 1) / VW;
   const long long target = 8LL * 1024;  // waves
   int H = 256;
@@ -150,14 +161,50 @@ int pick_band(int nx, int ny, int T) {
   return std::max(H, T);
 }
 
-void set_tiling(fdtd2d_plan *pl) {
+int set_tiling(fdtd2d_plan *pl) {
   const int T = pl->T > 0 ? pl->T : 4;
   const int halo = ((T + pl->C - 1) / pl->C) * pl->C;
   const int VW = 64 * pl->C - 2 * halo;
-  pl->a.ntx = (pl->L.nx + VW - 1) / VW;
-  pl->a.nwgx = (pl->a.ntx + fdtd2d::kWavesPerWG - 1) / fdtd2d::kWavesPerWG;
-  pl->a.H = pick_band(pl->L.nx, pl->L.ny, T);
-  pl->a.nty = (pl->L.ny + pl->a.H - 1) / pl->a.H;
+  fdtd2d::StepArgs &a = pl->a;
+  a.halo = halo;
+  a.vw = VW;
+  a.ntx = (pl->L.nx + VW - 1) / VW;
+  a.nwgx = (a.ntx + fdtd2d::kWavesPerWG - 1) / fdtd2d::kWavesPerWG;
+  a.H = pl->band > 0 ? std::max(pl->band, T) : pick_band(pl->L.nx, pl->L.ny, VW, T);
+  a.nty = (pl->L.ny + a.H - 1) / a.H;
+  // work-group lists: interior / CPML, each in (column group, band) order;
+  // CPML-class bands are cut into sub-bands of hc rows.
+  const int hc = pl->band_cpml > 0 ? std::max(pl->band_cpml, T) : std::max(16, T);
+  std::vector<int4> lists[2];
+  for (int wgx = 0; wgx < a.nwgx; ++wgx)
+    for (int ty = 0; ty < a.nty; ++ty) {
+      const int jb0 = ty * a.H, jb1 = std::min(pl->L.ny, jb0 + a.H);
+      if (!fdtd2d::wg_touches_cpml(a, T, wgx, jb0, jb1)) {
+        lists[0].push_back(make_int4(wgx, jb0, jb1, 0));
+        continue;
+      }
+      for (int r0 = jb0; r0 < jb1; r0 += hc) {
+        const int r1 = std::min(jb1, r0 + hc);
+        lists[fdtd2d::wg_touches_cpml(a, T, wgx, r0, r1) ? 1 : 0].push_back(make_int4(wgx, r0, r1, 0));
+      }
+    }
+  const size_t n = lists[0].size() + lists[1].size();
+  if (pl->wg_buf) {
+    hipFree(pl->wg_buf);
+    pl->wg_buf = nullptr;
+  }
+  HIP_TRY(hipMalloc(reinterpret_cast<void **>(&pl->wg_buf), std::max<size_t>(n, 1) * sizeof(int4)),
+          "hipMalloc tile lists");
+  HIP_TRY(hipMemcpy(pl->wg_buf, lists[0].data(), lists[0].size() * sizeof(int4), hipMemcpyHostToDevice),
+          "hipMemcpy lists");
+  HIP_TRY(hipMemcpy(pl->wg_buf + lists[0].size(), lists[1].data(), lists[1].size() * sizeof(int4),
+                    hipMemcpyHostToDevice),
+          "hipMemcpy lists");
+  a.wg_list[0] = pl->wg_buf;
+  a.wg_list[1] = pl->wg_buf + lists[0].size();
+  a.n_wg[0] = static_cast<int>(lists[0].size());
+  a.n_wg[1] = static_cast<int>(lists[1].size());
+  return FDTD2D_OK;
 }
 
 int current_T(const fdtd2d_plan *pl) { return pl->T > 0 ? pl->T : 4; }
@@ -228,6 +275,8 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   if (device < 0 || device >= ndev) return set_error(FDTD2D_EINVAL, "no such device");
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
 
+  if (static_cast<long long>(align_up(p->nx, 64)) * (p->ny + kTrashRows) * 4 > 0x7fffffffLL)
+    return set_error(FDTD2D_EUNSUPPORTED, "a field plane above 2 GiB (32-bit offsets)");
   std::vector<int32_t> ev_off;
   std::vector<fdtd2d::Event> events = build_events(p, ev_off);
   auto *pl = new (std::nothrow) fdtd2d_plan();
@@ -249,8 +298,12 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
     fdtd2d_plan_destroy(pl);
     return code;
   };
-  if (hipStreamCreateWithFlags(&pl->own_stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&pl->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&pl->aux, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(FDTD2D_EDEVICE, "hipStreamCreate"));
+  if (hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(set_error(FDTD2D_EDEVICE, "hipEventCreate"));
   pl->stream = pl->own_stream;
   if (hipMemset(pl->base, 0, L.total) != hipSuccess) return fail(set_error(FDTD2D_EDEVICE, "hipMemset"));
 
@@ -349,6 +402,17 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
     if ((rc = upload(pl, L.off_row[7], ezy_s.data(), ny * 4))) return fail(rc);
   }
   if ((rc = upload(pl, L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
+  {
+    std::vector<int4> ri(ny);
+    for (int j = 0; j < ny; ++j) {
+      const int jc = std::min(j, ny - 2);
+      int dh, de;
+      std::memcpy(&dh, &p->den_hy[jc], 4);
+      std::memcpy(&de, &p->den_ey[jc], 4);
+      ri[j] = make_int4(dh, de, ev_off[j], ev_off[j + 1]);
+    }
+    if ((rc = upload(pl, L.off_rowinfo, ri.data(), ri.size() * sizeof(int4)))) return fail(rc);
+  }
   if ((rc = upload(pl, L.off_events, events.data(), events.size() * sizeof(fdtd2d::Event)))) return fail(rc);
   if ((rc = upload(pl, L.off_src, p->src_table, static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4)))
     return fail(rc);
@@ -357,6 +421,7 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   a.nx = nx;
   a.ny = ny;
   a.pitch = pitch;
+  a.trash_rows = kTrashRows;
   a.P = P;
   a.xs = L.xs;
   a.nsteps = p->nsteps;
@@ -401,6 +466,7 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   a.hxy_slot = reinterpret_cast<const int32_t *>(pl->base + L.off_row[6]);
   a.ezy_slot = reinterpret_cast<const int32_t *>(pl->base + L.off_row[7]);
   a.ev_off = reinterpret_cast<const int32_t *>(pl->base + L.off_evoff);
+  a.rowinfo = reinterpret_cast<const int4 *>(pl->base + L.off_rowinfo);
   a.events = reinterpret_cast<const fdtd2d::Event *>(pl->base + L.off_events);
   a.src_table = reinterpret_cast<const float *>(pl->base + L.off_src);
   a.nsrc = p->nsrc;
@@ -409,7 +475,7 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   a.comp = reinterpret_cast<float *>(pl->base + L.off_comp);
   a.ncomp = p->ncomp;
   a.comp_plane = static_cast<long long>(p->nsteps + 1) * p->ncomp;
-  set_tiling(pl);
+  if ((rc = set_tiling(pl))) return fail(rc);
   *out = pl;
   return FDTD2D_OK;
 }
@@ -422,10 +488,38 @@ int fdtd2d_plan_set_stream(fdtd2d_plan *pl, void *stream) {
 
 int fdtd2d_plan_set_tblock(fdtd2d_plan *pl, int T) {
   if (!pl) return set_error(FDTD2D_EINVAL, "null plan");
-  if (T < 0 || T > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "steps per launch must be 0..4");
+  if (T < 0 || T > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "steps per launch must be 0..8");
   pl->T = T;
-  set_tiling(pl);
-  return FDTD2D_OK;
+  return set_tiling(pl);
+}
+
+int fdtd2d_plan_set_option(fdtd2d_plan *pl, const char *name, double value) {
+  if (!pl || !name) return set_error(FDTD2D_EINVAL, "null argument");
+  std::string n(name);
+  const int v = static_cast<int>(value);
+  if (n == "tblock") {
+    if (v < 0 || v > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "tblock out of range");
+    pl->T = v;
+  } else if (n == "band") {
+    if (v < 0 || v > pl->L.ny) return set_error(FDTD2D_EINVAL, "band out of range");
+    pl->band = v;
+  } else if (n == "band_cpml") {
+    if (v < 0 || v > pl->L.ny) return set_error(FDTD2D_EINVAL, "band_cpml out of range");
+    pl->band_cpml = v;
+  } else if (n == "cols_per_lane") {
+    if (v != 1 && v != 2 && v != 4) return set_error(FDTD2D_EINVAL, "cols_per_lane must be 1, 2 or 4");
+    pl->C = v;
+    if (v == 4) pl->D = 1;
+    if (v == 2 && pl->D > 2) pl->D = 2;
+  } else if (n == "prefetch") {
+    const int maxd = pl->C == 1 ? 4 : pl->C == 2 ? 2 : 1;
+    if (v != 1 && v != 2 && v != 4) return set_error(FDTD2D_EINVAL, "prefetch must be 1, 2 or 4");
+    if (v > maxd) return set_error(FDTD2D_EINVAL, "prefetch depth too large for cols_per_lane");
+    pl->D = v;
+  } else {
+    return set_error(FDTD2D_EINVAL, "unknown option " + n);
+  }
+  return set_tiling(pl);
 }
 
 int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value) {
@@ -436,10 +530,13 @@ int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value)
   else if (n == "band") *value = pl->a.H;
   else if (n == "ntx") *value = pl->a.ntx;
   else if (n == "nty") *value = pl->a.nty;
-  else if (n == "workgroups") *value = static_cast<double>(pl->a.nwgx) * pl->a.nty;
+  else if (n == "workgroups") *value = pl->a.n_wg[0] + pl->a.n_wg[1];
+  else if (n == "workgroups_cpml") *value = pl->a.n_wg[1];
+  else if (n == "tile_cols") *value = pl->a.vw;
   else if (n == "bytes") *value = static_cast<double>(pl->L.total);
   else if (n == "steps") *value = pl->n;
   else if (n == "cols_per_lane") *value = pl->C;
+  else if (n == "prefetch") *value = pl->D;
   else return set_error(FDTD2D_EINVAL, "unknown info name " + n);
   return FDTD2D_OK;
 }
@@ -459,8 +556,18 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       if (!e0 || !e1) return set_error(FDTD2D_EDEVICE, "hipEventCreate");
       HIP_TRY(hipEventRecord(e0, pl->stream), "hipEventRecord");
     }
-    hipError_t e = fdtd2d::launch_step(pl->a, t, pl->C, pl->src, pl->n, pl->stream);
+    // CPML work-groups on the aux stream, interior ones on the main stream;
+    // both read buffer set src and write the other, so they may overlap.
+    if (pl->a.n_wg[1] > 0) {
+      HIP_TRY(hipEventRecord(pl->ev_fork, pl->stream), "hipEventRecord fork");
+      HIP_TRY(hipStreamWaitEvent(pl->aux, pl->ev_fork, 0), "hipStreamWaitEvent fork");
+      hipError_t e = fdtd2d::launch_step(pl->a, 1, t, pl->C, 1, pl->src, pl->n, pl->aux);
+      if (e != hipSuccess) return hip_fail(e, "CPML step kernel launch");
+      HIP_TRY(hipEventRecord(pl->ev_join, pl->aux), "hipEventRecord join");
+    }
+    hipError_t e = fdtd2d::launch_step(pl->a, 0, t, pl->C, pl->D, pl->src, pl->n, pl->stream);
     if (e != hipSuccess) return hip_fail(e, "step kernel launch");
+    if (pl->a.n_wg[1] > 0) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
     if (pl->timing) {
       HIP_TRY(hipEventRecord(e1, pl->stream), "hipEventRecord");
       pl->pending.push_back({e0, e1});
@@ -626,7 +733,12 @@ int fdtd2d_plan_destroy(fdtd2d_plan *pl) {
     hipEventDestroy(pr.second);
   }
   for (auto e : pl->spare) hipEventDestroy(e);
+  if (pl->aux) hipStreamSynchronize(pl->aux);
+  if (pl->ev_fork) hipEventDestroy(pl->ev_fork);
+  if (pl->ev_join) hipEventDestroy(pl->ev_join);
+  if (pl->wg_buf) hipFree(pl->wg_buf);
   if (pl->base) hipFree(pl->base);
+  if (pl->aux) hipStreamDestroy(pl->aux);
   if (pl->own_stream) hipStreamDestroy(pl->own_stream);
   delete pl;
   return FDTD2D_OK;

@@ -25,6 +25,7 @@ struct Event {
 // recomputation in neighbouring tiles always sees the old values.
 struct StepArgs {
   int nx, ny, pitch;   // pitch: floats per row (multiple of 64), all fields
+  int trash_rows;      // scratch rows below row ny-1 of every field plane
   int P;               // CPML thickness (0 = none)
   int xs;              // floats per row of the x-strip psi arrays (2*P rounded up)
   int nsteps;
@@ -46,6 +47,8 @@ struct StepArgs {
   const int32_t *hyx_slot, *ezx_slot;          // [pitch], -1 outside the strips
   const float *hxy_b, *hxy_c, *ezy_b, *ezy_c;  // [ny]
   const int32_t *hxy_slot, *ezy_slot;          // [ny]
+  // per row: {den_hy bits, den_ey bits, first event, end event}, [ny]
+  const int4 *rowinfo;
   // events by row
   const int32_t *ev_off;  // [ny+1]
   const Event *events;
@@ -56,19 +59,46 @@ struct StepArgs {
   float *comp;
   int ncomp;
   long long comp_plane;
-  // tiling
+  // tiling (fixed per plan; launches of fewer steps reuse the same tiles)
+  int vw;        // stored columns per tile = 64*C - 2*halo
+  int halo;      // recomputed columns on each side, >= steps per launch
   int H;         // rows per band
   int ntx, nty;  // tiles across / down
   int nwgx;      // work-groups across (4 tiles each)
-  const int32_t *tile_class;  // [nty * nwgx]: 1 if any tile of the WG touches a CPML strip
+  // Work-group lists, entry = {column group, first row, end row, 0}:
+  // [0] interior work-groups (no tile loads a CPML cell) on bands of H rows,
+  // [1] the rest, on shorter bands so that they finish well inside the
+  // interior launch they run beside. Each list is one launch.
+  const int4 *wg_list[2];
+  int n_wg[2];
 };
 
 // One launch advances T steps (1..kMaxT).
-constexpr int kMaxT = 4;
+constexpr int kMaxT = 8;
 constexpr int kWavesPerWG = 4;
 
-// Launches the step kernel for T steps on `stream`; src = buffer set read.
-hipError_t launch_step(const StepArgs &a, int T, int C, int src, int n0, hipStream_t stream);
+// Launches the step kernel of work-group class `cls` (0 interior, 1 CPML)
+// for T steps on `stream`, D rows of prefetch; src = buffer set read.
+hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream);
+
+// True when any tile of a work-group (column group wgx, rows [jb0, jb1))
+// loads a cell of a CPML strip.
+// (With no CPML the outer PEC ring counts as the strip: the interior kernel
+// applies no index masks, so it must never load a ring cell.)
+inline bool wg_touches_cpml(const StepArgs &a, int T, int wgx, int jb0, int jb1) {
+  const int P = a.P > 0 ? a.P : 1;
+  const int halo = a.halo;
+  const int VW = a.vw;
+  const int jl0 = jb0 - T > 0 ? jb0 - T : 0, jl1 = jb1 + T < a.ny ? jb1 + T : a.ny;
+  if (jl0 <= P || jl1 >= a.ny - P - 1) return true;
+  for (int w = 0; w < kWavesPerWG; ++w) {
+    const int tx = wgx * kWavesPerWG + w;
+    if (tx >= a.ntx) break;
+    const int cs = tx * VW - halo, ce = cs + VW + 2 * halo;
+    if (cs <= P || ce >= a.nx - P - 1) return true;
+  }
+  return false;
+}
 
 }  // namespace fdtd2d
 

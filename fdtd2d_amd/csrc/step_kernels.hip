@@ -1,9 +1,9 @@
 // step_kernels.hip - fused, temporally blocked TMz step kernels for gfx950.
 //
-// One launch advances T (1..4) time steps. Mapping:
+// One launch advances T (1..8) time steps. Mapping:
 //  * a wavefront owns a tile: a strip of 64*C consecutive columns (C per
-//    lane) and a band of H rows. Its outer HALO = roundup(T, C) columns on
-//    each side are recomputed, not stored (VW = 64*C - 2*HALO are stored);
+//    lane) and a band of rows. Its outer HALO >= T columns on each side are
+//    recomputed, not stored (VW = 64*C - 2*HALO are stored);
 //  * the wave streams down its band one row per iteration. Level t (time step
 //    n0+t) runs one row behind level t-1, so every level finds the rows it
 //    needs in registers: Ez^{t-1} at rows r and r+1, Hx^{t-1/2} at r-1; the
@@ -13,7 +13,13 @@
 //    stops it;
 //  * all state (Ez, Hx, Hy, Jz, the psi strips) is ping-ponged between two
 //    buffer sets so that halo recomputation in neighbouring tiles reads the
-//    step-n0 values while the owner writes step n0+T.
+//    step-n0 values while the owner writes step n0+T;
+//  * two kernels share a launch: EDGE work-groups (any tile loads a cell of a
+//    CPML strip or of the outer ring) carry the psi state and the reference's
+//    index ranges; interior work-groups carry neither, so their per-cell code
+//    is straight-line (no lane-divergent branches), and they address HBM
+//    with one 32-bit row offset shared by all streams - the interior loop is
+//    otherwise limited by scalar-unit issue.
 // Per cell and step the arithmetic is that of the reference kernels
 // (F/kernel.cu:127-537, abbreviated as in fdtd2d.h), in the same order and
 // without FMA contraction, so results match the CPU oracle bit for bit:
@@ -36,11 +42,21 @@ namespace {
 constexpr int kTable = 0;
 constexpr int kDensity = 1;
 
-template <int C, bool PSI>
+// Read-only tables (written by the host before any launch) are read through
+// the constant address space, so uniform indices become scalar loads; plain
+// global pointers would compile to vector loads plus s_waitcnt vmcnt(0),
+// which also waits for the row prefetches in flight.
+typedef const int __attribute__((address_space(4))) *cint_ptr;
+typedef const float __attribute__((address_space(4))) *cfloat_ptr;
+__device__ __forceinline__ int cld(const int *p, long long i) { return ((cint_ptr)p)[i]; }
+__device__ __forceinline__ float cld(const float *p, long long i) { return ((cfloat_ptr)p)[i]; }
+__device__ __forceinline__ int cld_i(const void *p, long long i) { return ((cint_ptr)p)[i]; }
+
+template <int C, bool EDGE>
 struct Level {
   float ez[C], hx[C], hy[C], jz[C];
   uint32_t med[C];  // TABLE: material id; DENSITY: bits of Ne
-  float phxy[PSI ? C : 1], pezy[PSI ? C : 1], phyx[PSI ? C : 1], pezx[PSI ? C : 1];
+  float phxy[EDGE ? C : 1], pezy[EDGE ? C : 1], phyx[EDGE ? C : 1], pezx[EDGE ? C : 1];
 };
 
 __device__ __forceinline__ float jz_update(float jz, float decay, float cj, float ez, float ne,
@@ -51,37 +67,61 @@ __device__ __forceinline__ float jz_update(float jz, float decay, float cj, floa
 }
 
 // Column constants of one lane.
-template <int C, bool PSI>
+template <int C, bool EDGE>
 struct Cols {
   int c0;
   float den_ex[C], den_hx[C];
-  float hyx_b[PSI ? C : 1], hyx_c[PSI ? C : 1], ezx_b[PSI ? C : 1], ezx_c[PSI ? C : 1];
-  int hyx_s[PSI ? C : 1], ezx_s[PSI ? C : 1];
+  float hyx_b[EDGE ? C : 1], hyx_c[EDGE ? C : 1], ezx_b[EDGE ? C : 1], ezx_c[EDGE ? C : 1];
+  int hyx_s[EDGE ? C : 1], ezx_s[EDGE ? C : 1];
 };
 
-template <int C, bool PSI, int MED>
-__device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, const Cols<C, PSI> &cl,
-                                        const Level<C, PSI> &p, const float (&ez_s)[C],
-                                        const float (&hx_n)[C], int r, int step, int own0, int own1,
-                                        bool own_row, Level<C, PSI> &o) {
-  const int nx = a.nx, ny = a.ny;
-  const bool rowH = r >= 0 && r <= ny - 2;
-  const bool rowE = r >= 1 && r <= ny - 2;
-  const int rc = min(max(r, 0), ny - 2);
-  const float den_hy = a.den_hy[rc];
-  const float den_ey = a.den_ey[rc];
-  int hxy_s = -1, ezy_s = -1;
-  float hxy_b = 0.f, hxy_c = 0.f, ezy_b = 0.f, ezy_c = 0.f;
-  if constexpr (PSI) {
-    if (rowH) {
-      hxy_s = a.hxy_slot[r];
-      ezy_s = a.ezy_slot[r];
-      hxy_b = a.hxy_b[r];
-      hxy_c = a.hxy_c[r];
-      ezy_b = a.ezy_b[r];
-      ezy_c = a.ezy_c[r];
+// Per-row scalars, loaded once when the row enters the pipeline and carried
+// down the levels with it (they are uniform across the wave).
+template <bool EDGE>
+struct RowS {
+  float den_hy, den_ey;
+  int e0, e1;
+  int hxy_s, ezy_s;
+  float hxy_b, hxy_c, ezy_b, ezy_c;
+};
+
+template <bool EDGE>
+__device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
+  RowS<EDGE> q;
+  const bool in = r >= 0 && r < a.ny;
+  const long long ri = 4LL * (in ? r : 0);
+  q.den_hy = __int_as_float(cld_i(a.rowinfo, ri));
+  q.den_ey = __int_as_float(cld_i(a.rowinfo, ri + 1));
+  const int e0 = cld_i(a.rowinfo, ri + 2), e1 = cld_i(a.rowinfo, ri + 3);
+  q.e0 = in ? e0 : 0;
+  q.e1 = in ? e1 : 0;
+  q.hxy_s = q.ezy_s = -1;
+  q.hxy_b = q.hxy_c = q.ezy_b = q.ezy_c = 0.f;
+  if constexpr (EDGE) {
+    if (r >= 0 && r <= a.ny - 2) {
+      q.hxy_s = cld(a.hxy_slot, r);
+      q.ezy_s = cld(a.ezy_slot, r);
+      q.hxy_b = cld(a.hxy_b, r);
+      q.hxy_c = cld(a.hxy_c, r);
+      q.ezy_b = cld(a.ezy_b, r);
+      q.ezy_c = cld(a.ezy_c, r);
     }
   }
+  return q;
+}
+
+// One level: level t-1's row r (p), Ez^{t-1}(r+1) (ez_s) and Hx^{t-1/2}(r-1)
+// (hx_n) -> level t's row r (o).
+template <int C, bool EDGE, int MED>
+__device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl,
+                                        const RowS<EDGE> &rs, const Level<C, EDGE> &p, const float (&ez_s)[C],
+                                        const float (&hx_n)[C], int r, int step, int own0, int own1,
+                                        bool own_row, Level<C, EDGE> &o) {
+  const int nx = a.nx, ny = a.ny;
+  // Index ranges of the reference kernels; interior tiles lie inside all of
+  // them by construction, so there the masks are compile-time true.
+  const bool rowH = !EDGE || (r >= 0 && r <= ny - 2);
+  const bool rowE = !EDGE || (r >= 1 && r <= ny - 2);
   // Ez^{t-1}(r, i+1)
   float ez_e[C];
 #pragma unroll
@@ -91,19 +131,18 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const int col = cl.c0 + c;
-    const bool updH = rowH && col >= 0 && col <= nx - 2;
-    float hx = p.hx[c], hy = p.hy[c];
+    const bool updH = !EDGE || (rowH && col >= 0 && col <= nx - 2);
     const float dEy = p.ez[c] - ez_s[c];
     const float dEx = ez_e[c] - p.ez[c];
-    if (updH) {
-      hx = hx + a.db * (dEy * den_hy);
-      hy = hy + a.db * (dEx * cl.den_hx[c]);
-    }
-    if constexpr (PSI) {
+    float hx = p.hx[c] + a.db * (dEy * rs.den_hy);
+    float hy = p.hy[c] + a.db * (dEx * cl.den_hx[c]);
+    if constexpr (EDGE) {
       o.phxy[c] = p.phxy[c];
       o.phyx[c] = p.phyx[c];
-      if (updH && hxy_s >= 0) {
-        const float ps = hxy_b * p.phxy[c] + (hxy_c * dEy) / a.dyf;
+      hx = updH ? hx : p.hx[c];
+      hy = updH ? hy : p.hy[c];
+      if (updH && rs.hxy_s >= 0) {
+        const float ps = rs.hxy_b * p.phxy[c] + (rs.hxy_c * dEy) / a.dyf;
         o.phxy[c] = ps;
         hx = hx + a.db * ps;
       }
@@ -125,7 +164,7 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const int col = cl.c0 + c;
-    const bool updE = rowE && col >= 1 && col <= nx - 2;
+    const bool updE = !EDGE || (rowE && col >= 1 && col <= nx - 2);
     float ca, cb, ne;
     bool pec = false;
     if constexpr (MED == kTable) {
@@ -139,18 +178,14 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
       cb = a.cb0;
       ne = __uint_as_float(p.med[c]);
     }
-    float jz = p.jz[c];
-    if (updE && ne > 0.f) jz = jz_update(jz, a.jz_decay, a.cj, p.ez[c], ne, a.jz_dt);
-    float ez = p.ez[c];
+    const float jz_new = jz_update(p.jz[c], a.jz_decay, a.cj, p.ez[c], ne, a.jz_dt);
+    const float jz = (updE && ne > 0.f) ? jz_new : p.jz[c];
     const float dHy = o.hy[c] - hy_w[c];
     const float dHx = hx_n[c] - o.hx[c];
-    if (updE) {
-      if (pec)
-        ez = 0.f;
-      else
-        ez = ca * ez + cb * ((dHy * cl.den_ex[c] + dHx * den_ey) - jz);
-    }
-    if constexpr (PSI) {
+    float ez = ca * p.ez[c] + cb * ((dHy * cl.den_ex[c] + dHx * rs.den_ey) - jz);
+    if constexpr (MED == kTable) ez = pec ? 0.f : ez;
+    if constexpr (EDGE) {
+      ez = updE ? ez : p.ez[c];
       o.pezx[c] = p.pezx[c];
       o.pezy[c] = p.pezy[c];
       if (updE && cl.ezx_s[c] >= 0) {
@@ -158,8 +193,8 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
         o.pezx[c] = ps;
         ez = ez + cb * ps;
       }
-      if (updE && ezy_s >= 0) {
-        const float ps = ezy_b * p.pezy[c] + (ezy_c * dHx) / a.dyf;
+      if (updE && rs.ezy_s >= 0) {
+        const float ps = rs.ezy_b * p.pezy[c] + (rs.ezy_c * dHx) / a.dyf;
         o.pezy[c] = ps;
         ez = ez + cb * ps;
       }
@@ -169,27 +204,65 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
     o.med[c] = p.med[c];
   }
   // ---- soft sources, then detector samples of this step (F/kernel.cu:621-805)
-  if (r >= 0 && r < ny) {
-    const int e0 = a.ev_off[r], e1 = a.ev_off[r + 1];
-    for (int e = e0; e < e1; ++e) {
-      const Event ev = a.events[e];
+  for (int e = rs.e0; e < rs.e1; ++e) {
+    Event ev;
+    ev.i = cld_i(a.events, 4LL * e);
+    ev.kind = cld_i(a.events, 4LL * e + 1);
+    ev.idx = cld_i(a.events, 4LL * e + 2);
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        if (cl.c0 + c != ev.i) continue;
-        if (ev.kind == EV_SRC) {
-          o.ez[c] = o.ez[c] - a.src_table[static_cast<long long>(step) * a.nsrc + ev.idx];
-        } else if (own_row && ev.i >= own0 && ev.i < own1) {
-          if (ev.kind == EV_DET) {
-            a.det[static_cast<long long>(step) * a.ndet + ev.idx] = o.ez[c];
-          } else {
-            const long long q = static_cast<long long>(step) * a.ncomp + ev.idx;
-            a.comp[q] = o.ez[c];
-            a.comp[a.comp_plane + q] = o.hx[c];
-            a.comp[2 * a.comp_plane + q] = o.hy[c];
-          }
+    for (int c = 0; c < C; ++c) {
+      if (cl.c0 + c != ev.i) continue;
+      if (ev.kind == EV_SRC) {
+        o.ez[c] = o.ez[c] - cld(a.src_table, static_cast<long long>(step) * a.nsrc + ev.idx);
+      } else if (own_row && ev.i >= own0 && ev.i < own1) {
+        if (ev.kind == EV_DET) {
+          a.det[static_cast<long long>(step) * a.ndet + ev.idx] = o.ez[c];
+        } else {
+          const long long q = static_cast<long long>(step) * a.ncomp + ev.idx;
+          a.comp[q] = o.ez[c];
+          a.comp[a.comp_plane + q] = o.hx[c];
+          a.comp[2 * a.comp_plane + q] = o.hy[c];
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------- memory
+// Interior tiles address every stream as (uniform base pointer) + (32-bit
+// byte offset shared by all streams): one VGPR offset per row, 2 SGPRs per
+// stream, and the loads take the global_load v, voff, s[base] form.
+struct Streams {
+  const char *ez, *hx, *hy, *jz, *med;
+};
+
+template <int C>
+__device__ __forceinline__ void oload(const char *base, unsigned off, float (&v)[C]) {
+  const char *q = base + off;
+  if constexpr (C == 1) {
+    v[0] = *reinterpret_cast<const float *>(q);
+  } else if constexpr (C == 2) {
+    const float2 t = *reinterpret_cast<const float2 *>(q);
+    v[0] = t.x;
+    v[1] = t.y;
+  } else {
+    const float4 t = *reinterpret_cast<const float4 *>(q);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+  }
+}
+
+template <int C>
+__device__ __forceinline__ void ostore(const char *base, unsigned off, const float (&v)[C]) {
+  char *q = const_cast<char *>(base) + off;
+  if constexpr (C == 1) {
+    *reinterpret_cast<float *>(q) = v[0];
+  } else if constexpr (C == 2) {
+    *reinterpret_cast<float2 *>(q) = make_float2(v[0], v[1]);
+  } else {
+    *reinterpret_cast<float4 *>(q) = make_float4(v[0], v[1], v[2], v[3]);
   }
 }
 
@@ -217,29 +290,172 @@ __device__ __forceinline__ void load_row(const float *__restrict__ base, int k, 
   }
 }
 
-template <int T, int C, bool PSI, int MED>
+// Stores the C values of a lane at columns [c0, c0+C) of row r, only those in
+// [own0, own1): one vector store when all are owned.
+template <int C>
+__device__ __forceinline__ void store_row(float *__restrict__ base, long long rowp, int c0, int own0, int own1,
+                                          const float (&v)[C]) {
+  float *q = base + rowp + c0;
+  if (c0 >= own0 && c0 + C <= own1) {
+    if constexpr (C == 1) {
+      q[0] = v[0];
+    } else if constexpr (C == 2) {
+      *reinterpret_cast<float2 *>(q) = make_float2(v[0], v[1]);
+    } else {
+      *reinterpret_cast<float4 *>(q) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+      if (c0 + c >= own0 && c0 + c < own1) q[c] = v[c];
+  }
+}
+
+// Level 0 of row k: the step-n0 state (zeros for rows outside [0, ny) or
+// past the band's last loaded row).
+template <int C, bool EDGE, int MED>
+__device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm, const Cols<C, EDGE> &cl,
+                                            int src, int k, bool rv, bool lane_in, Level<C, EDGE> &in) {
+  const int pitch = a.pitch;
+  if constexpr (!EDGE) {
+    // interior: every lane's columns are inside the grid. The loads are
+    // unconditional (the caller clamps k to the band's last loaded row; rows
+    // past it lie outside the dependency cone of the stored rows), so every
+    // iteration issues the same memory operations and the wait before the
+    // next row's use covers exactly that row's loads.
+    (void)rv;
+    const unsigned off = static_cast<unsigned>(cl.c0) * 4u + static_cast<unsigned>(k) * static_cast<unsigned>(pitch) * 4u;
+    float mv[C];
+    oload<C>(sm.ez, off, in.ez);
+    oload<C>(sm.hx, off, in.hx);
+    oload<C>(sm.hy, off, in.hy);
+    oload<C>(sm.jz, off, in.jz);
+    oload<C>(sm.med, off, mv);
+#pragma unroll
+    for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
+  } else {
+    const bool ld = rv && lane_in;
+    load_row<C>(a.ez[src], k, cl.c0, pitch, ld, in.ez);
+    load_row<C>(a.hx[src], k, cl.c0, pitch, ld, in.hx);
+    load_row<C>(a.hy[src], k, cl.c0, pitch, ld, in.hy);
+    load_row<C>(a.jz[src], k, cl.c0, pitch, ld, in.jz);
+    {
+      float mv[C];
+      load_row<C>(MED == kTable ? reinterpret_cast<const float *>(a.med_id) : a.med_ne, k, cl.c0, pitch, ld, mv);
+#pragma unroll
+      for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
+    }
+    const int sh = rv ? a.hxy_slot[k] : -1;
+    const int se = rv ? a.ezy_slot[k] : -1;
+    if (sh >= 0)
+      load_row<C>(a.psi_hxy[src] + static_cast<long long>(sh) * pitch, 0, cl.c0, pitch, lane_in, in.phxy);
+    else
+#pragma unroll
+      for (int c = 0; c < C; ++c) in.phxy[c] = 0.f;
+    if (se >= 0)
+      load_row<C>(a.psi_ezy[src] + static_cast<long long>(se) * pitch, 0, cl.c0, pitch, lane_in, in.pezy);
+    else
+#pragma unroll
+      for (int c = 0; c < C; ++c) in.pezy[c] = 0.f;
+    const long long rowx = static_cast<long long>(k) * a.xs;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      in.phyx[c] = (rv && cl.hyx_s[c] >= 0) ? a.psi_hyx[src][rowx + cl.hyx_s[c]] : 0.f;
+      in.pezx[c] = (rv && cl.ezx_s[c] >= 0) ? a.psi_ezx[src][rowx + cl.ezx_s[c]] : 0.f;
+    }
+  }
+}
+
+// One row iteration: level 0 takes row k, level t computes row k-t, and
+// row k-T is stored.
+template <int T, int C, bool EDGE, int MED>
+__device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, const float4 *lut_s,
+                                         const Cols<C, EDGE> &cl, int src, int n0, int k, int jb0, int jb1,
+                                         int own0, int own1, bool lane_owns, unsigned trash_off,
+                                         const Level<C, EDGE> &in, Level<C, EDGE> (&st)[T + 1],
+                                         RowS<EDGE> (&rs)[T + 1]) {
+#pragma unroll
+  for (int t = T; t >= 1; --t) rs[t] = rs[t - 1];
+  rs[0] = row_scalars<EDGE>(a, k);
+  Level<C, EDGE> prev = st[0];
+  st[0] = in;
+  float nxt_ez[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) nxt_ez[c] = in.ez[c];
+  // ---- levels 1..T, each one row behind the previous
+#pragma unroll
+  for (int t = 1; t <= T; ++t) {
+    const int r = k - t;
+    Level<C, EDGE> out;
+    const bool own_row = r >= jb0 && r < jb1;
+    advance<C, EDGE, MED>(a, lut_s, cl, rs[t], prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
+    prev = st[t];
+    st[t] = out;
+#pragma unroll
+    for (int c = 0; c < C; ++c) nxt_ez[c] = out.ez[c];
+  }
+  // ---- store step n0+T for owned cells of row k-T
+  const int r = k - T;
+  if constexpr (!EDGE) {
+    // Unconditional stores (see load_level0): lanes or rows this tile does
+    // not own write this wave's scratch slot below the grid instead.
+    const bool ok = lane_owns && r >= jb0 && r < jb1;
+    const unsigned off = ok ? static_cast<unsigned>(cl.c0) * 4u +
+                                  static_cast<unsigned>(r) * static_cast<unsigned>(a.pitch) * 4u
+                            : trash_off;
+    const Level<C, EDGE> &o = st[T];
+    ostore<C>(dst.ez, off, o.ez);
+    ostore<C>(dst.hx, off, o.hx);
+    ostore<C>(dst.hy, off, o.hy);
+    ostore<C>(dst.jz, off, o.jz);
+  } else if (r >= jb0 && r < jb1) {
+    const int pitch = a.pitch;
+    const Level<C, EDGE> &o = st[T];
+    {
+      const long long rowp = static_cast<long long>(r) * pitch;
+      store_row<C>(a.ez[1 - src], rowp, cl.c0, own0, own1, o.ez);
+      store_row<C>(a.hx[1 - src], rowp, cl.c0, own0, own1, o.hx);
+      store_row<C>(a.hy[1 - src], rowp, cl.c0, own0, own1, o.hy);
+      store_row<C>(a.jz[1 - src], rowp, cl.c0, own0, own1, o.jz);
+      const int sh = rs[T].hxy_s, se = rs[T].ezy_s;
+      if (sh >= 0) store_row<C>(a.psi_hxy[1 - src], static_cast<long long>(sh) * pitch, cl.c0, own0, own1, o.phxy);
+      if (se >= 0) store_row<C>(a.psi_ezy[1 - src], static_cast<long long>(se) * pitch, cl.c0, own0, own1, o.pezy);
+      const long long rowx = static_cast<long long>(r) * a.xs;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int col = cl.c0 + c;
+        if (col < own0 || col >= own1) continue;
+        if (cl.hyx_s[c] >= 0) a.psi_hyx[1 - src][rowx + cl.hyx_s[c]] = o.phyx[c];
+        if (cl.ezx_s[c] >= 0) a.psi_ezx[1 - src][rowx + cl.ezx_s[c]] = o.pezx[c];
+      }
+    }
+  }
+}
+
+// D = rows fetched ahead of the one being consumed (memory-level
+// parallelism per wave); the row loop is unrolled by D so the prefetch ring
+// is indexed statically.
+template <int T, int C, int D, bool EDGE, int MED>
 __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s, int src, int n0,
-                                         int tx, int ty) {
-  constexpr int HALO = ((T + C - 1) / C) * C;
-  constexpr int VW = 64 * C - 2 * HALO;
+                                         int tx, int jb0, int jb1) {
+  const int HALO = a.halo;  // >= T, a multiple of C
+  const int VW = a.vw;      // 64*C - 2*HALO
   const int lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny, pitch = a.pitch;
   const int cs = tx * VW - HALO;
-  Cols<C, PSI> cl;
+  Cols<C, EDGE> cl;
   cl.c0 = cs + lane * C;
   const int own0 = tx * VW, own1 = min(nx, own0 + VW);
-  const int jb0 = ty * a.H, jb1 = min(ny, jb0 + a.H);
   const int jl0 = max(0, jb0 - T), jl1 = min(ny, jb1 + T);
   const bool lane_in = cl.c0 >= 0 && cl.c0 < pitch;  // the lane's C columns are in memory
-  const float *ezA = a.ez[src], *hxA = a.hx[src], *hyA = a.hy[src], *jzA = a.jz[src];
-  float *ezB = a.ez[1 - src], *hxB = a.hx[1 - src], *hyB = a.hy[1 - src], *jzB = a.jz[1 - src];
+  const bool lane_owns = cl.c0 >= own0 && cl.c0 + C <= own1;
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const int col = cl.c0 + c;
     const bool in = col >= 0 && col < pitch;
     cl.den_ex[c] = in ? a.den_ex[col] : 0.f;
     cl.den_hx[c] = in ? a.den_hx[col] : 0.f;
-    if constexpr (PSI) {
+    if constexpr (EDGE) {
       cl.hyx_s[c] = in ? a.hyx_slot[col] : -1;
       cl.ezx_s[c] = in ? a.ezx_slot[col] : -1;
       cl.hyx_b[c] = in ? a.hyx_b[col] : 0.f;
@@ -248,150 +464,125 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       cl.ezx_c[c] = in ? a.ezx_c[col] : 0.f;
     }
   }
-  Level<C, PSI> st[T + 1];
+  Streams sm, dst;
+  sm.ez = reinterpret_cast<const char *>(a.ez[src]);
+  sm.hx = reinterpret_cast<const char *>(a.hx[src]);
+  sm.hy = reinterpret_cast<const char *>(a.hy[src]);
+  sm.jz = reinterpret_cast<const char *>(a.jz[src]);
+  sm.med = MED == kTable ? reinterpret_cast<const char *>(a.med_id) : reinterpret_cast<const char *>(a.med_ne);
+  dst.ez = reinterpret_cast<const char *>(a.ez[1 - src]);
+  dst.hx = reinterpret_cast<const char *>(a.hx[1 - src]);
+  dst.hy = reinterpret_cast<const char *>(a.hy[1 - src]);
+  dst.jz = reinterpret_cast<const char *>(a.jz[1 - src]);
+  dst.med = sm.med;
+  Level<C, EDGE> st[T + 1];
+  RowS<EDGE> rs[T + 1];
 #pragma unroll
-  for (int t = 0; t <= T; ++t)
+  for (int t = 0; t <= T; ++t) {
+    rs[t] = row_scalars<EDGE>(a, -1);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       st[t].ez[c] = st[t].hx[c] = st[t].hy[c] = st[t].jz[c] = 0.f;
       st[t].med[c] = 0u;
-      if constexpr (PSI) st[t].phxy[c] = st[t].pezy[c] = st[t].phyx[c] = st[t].pezx[c] = 0.f;
+      if constexpr (EDGE) st[t].phxy[c] = st[t].pezy[c] = st[t].phyx[c] = st[t].pezx[c] = 0.f;
     }
-
-  for (int k = jl0; k < jl1 + T; ++k) {
-    // ---- level 0: row k of the step-n0 state
-    Level<C, PSI> in;
-    const bool rv = k < jl1;
-    const bool ld = rv && lane_in;
-    load_row<C>(ezA, k, cl.c0, pitch, ld, in.ez);
-    load_row<C>(hxA, k, cl.c0, pitch, ld, in.hx);
-    load_row<C>(hyA, k, cl.c0, pitch, ld, in.hy);
-    load_row<C>(jzA, k, cl.c0, pitch, ld, in.jz);
-    {
-      float mv[C];
-      load_row<C>(MED == kTable ? reinterpret_cast<const float *>(a.med_id) : a.med_ne, k, cl.c0,
-                  pitch, ld, mv);
+  }
+  // Scratch slot of this wave in the rows below the grid (see row_step).
+  const unsigned nslot = static_cast<unsigned>(a.trash_rows * pitch / (64 * C));
+  const unsigned slot = (blockIdx.x * kWavesPerWG + (threadIdx.x >> 6)) % nslot;
+  const unsigned trash_off =
+      (static_cast<unsigned>(ny) * static_cast<unsigned>(pitch) + slot * 64u * C + static_cast<unsigned>(lane * C)) * 4u;
+  // Drain the loop-invariant loads above: otherwise the row loop's wait
+  // counts cannot separate them from the prefetches and wait for everything.
+  __builtin_amdgcn_s_waitcnt(0);
+  Level<C, EDGE> pre[D];
 #pragma unroll
-      for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
-    }
-    if constexpr (PSI) {
-      const int sh = rv ? a.hxy_slot[k] : -1;
-      const int se = rv ? a.ezy_slot[k] : -1;
-      if (sh >= 0)
-        load_row<C>(a.psi_hxy[src] + static_cast<long long>(sh) * pitch, 0, cl.c0, pitch, lane_in, in.phxy);
-      else
+  for (int u = 0; u < D; ++u) {
+    const int kk = EDGE ? jl0 + u : min(jl0 + u, jl1 - 1);
+    load_level0<C, EDGE, MED>(a, sm, cl, src, kk, jl0 + u < jl1, lane_in, pre[u]);
+  }
+  // Iterations k in [jl0, jl1 + T), rounded up to a multiple of D: the extra
+  // rows lie below the band's output rows and store nothing.
+  for (int k = jl0; k < jl1 + T; k += D) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) in.phxy[c] = 0.f;
-      if (se >= 0)
-        load_row<C>(a.psi_ezy[src] + static_cast<long long>(se) * pitch, 0, cl.c0, pitch, lane_in, in.pezy);
-      else
-#pragma unroll
-        for (int c = 0; c < C; ++c) in.pezy[c] = 0.f;
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const long long rowx = static_cast<long long>(k) * a.xs;
-        in.phyx[c] = (rv && cl.hyx_s[c] >= 0) ? a.psi_hyx[src][rowx + cl.hyx_s[c]] : 0.f;
-        in.pezx[c] = (rv && cl.ezx_s[c] >= 0) ? a.psi_ezx[src][rowx + cl.ezx_s[c]] : 0.f;
-      }
-    }
-    Level<C, PSI> prev = st[0];
-    st[0] = in;
-    float nxt_ez[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) nxt_ez[c] = in.ez[c];
-    // ---- levels 1..T, each one row behind the previous
-#pragma unroll
-    for (int t = 1; t <= T; ++t) {
-      const int r = k - t;
-      Level<C, PSI> out;
-      const bool own_row = r >= jb0 && r < jb1;
-      advance<C, PSI, MED>(a, lut_s, cl, prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
-      prev = st[t];
-      st[t] = out;
-#pragma unroll
-      for (int c = 0; c < C; ++c) nxt_ez[c] = out.ez[c];
-    }
-    // ---- store step n0+T for owned cells of row k-T
-    const int r = k - T;
-    if (r >= jb0 && r < jb1) {
-      const long long rowp = static_cast<long long>(r) * pitch;
-      const Level<C, PSI> &o = st[T];
-      int sh = -1, se = -1;
-      if constexpr (PSI) {
-        sh = a.hxy_slot[r];
-        se = a.ezy_slot[r];
-      }
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int col = cl.c0 + c;
-        if (col < own0 || col >= own1) continue;
-        ezB[rowp + col] = o.ez[c];
-        hxB[rowp + col] = o.hx[c];
-        hyB[rowp + col] = o.hy[c];
-        jzB[rowp + col] = o.jz[c];
-        if constexpr (PSI) {
-          if (sh >= 0) a.psi_hxy[1 - src][static_cast<long long>(sh) * pitch + col] = o.phxy[c];
-          if (se >= 0) a.psi_ezy[1 - src][static_cast<long long>(se) * pitch + col] = o.pezy[c];
-          const long long rowx = static_cast<long long>(r) * a.xs;
-          if (cl.hyx_s[c] >= 0) a.psi_hyx[1 - src][rowx + cl.hyx_s[c]] = o.phyx[c];
-          if (cl.ezx_s[c] >= 0) a.psi_ezx[1 - src][rowx + cl.ezx_s[c]] = o.pezx[c];
-        }
-      }
+    for (int u = 0; u < D; ++u) {
+      const Level<C, EDGE> in = pre[u];
+      const int kn = k + u + D;
+      const int kk = EDGE ? kn : min(kn, jl1 - 1);
+      load_level0<C, EDGE, MED>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
+      row_step<T, C, EDGE, MED>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns, trash_off, in,
+                                st, rs);
     }
   }
 }
 
-template <int T, int C, int MED>
+template <int T, int C, int D, int MED, bool EDGE>
 __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a, int src, int n0) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   if constexpr (MED == kTable) {
     for (int q = threadIdx.x; q < a.nmat; q += blockDim.x) lut_s[q] = a.lut[q];
     __syncthreads();
   }
-  // Work-group -> (column group, band): consecutive work-groups of one XCD
-  // (blockIdx % 8 round robin) take vertically adjacent bands so that the
-  // band halos they share stay in that XCD's L2.
+  // Launch index -> list position: consecutive positions go to one XCD
+  // (work-groups are dealt to the 8 XCDs round robin), and the lists are
+  // ordered by column group then band, so an XCD works down vertically
+  // adjacent bands whose shared halo rows stay in its L2.
   const int G = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, q = G >> 3, rr = G & 7;
-  const int L = xcd * q + min(xcd, rr) + (b >> 3);
-  const int ty = L % a.nty;
-  const int wgx = L / a.nty;
-  const int tx = wgx * kWavesPerWG + (threadIdx.x >> 6);
+  const int pos = xcd * q + min(xcd, rr) + (b >> 3);
+  const int4 w = a.wg_list[EDGE ? 1 : 0][pos];
+  const int tx = w.x * kWavesPerWG + (threadIdx.x >> 6);
   if (tx >= a.ntx) return;
-  constexpr int HALO = ((T + C - 1) / C) * C;
-  constexpr int VW = 64 * C - 2 * HALO;
-  bool psi = false;
-  if (a.P > 0) {
-    const int cs = tx * VW - HALO, ce = cs + 64 * C;  // loaded columns [cs, ce)
-    const int jb0 = ty * a.H, jb1 = min(a.ny, jb0 + a.H);
-    const int jl0 = max(0, jb0 - T), jl1 = min(a.ny, jb1 + T);
-    psi = cs <= a.P || ce >= a.nx - a.P - 1 || jl0 <= a.P || jl1 >= a.ny - a.P - 1;
-  }
-  if (psi)
-    run_tile<T, C, true, MED>(a, lut_s, src, n0, tx, ty);
-  else
-    run_tile<T, C, false, MED>(a, lut_s, src, n0, tx, ty);
+  run_tile<T, C, D, EDGE, MED>(a, lut_s, src, n0, tx, w.y, w.z);
 }
 
-template <int T, int C>
-hipError_t launch_tc(const StepArgs &a, int src, int n0, hipStream_t s) {
-  const int grid = a.nwgx * a.nty;
+template <int T, int C, int D, bool EDGE>
+hipError_t launch_tcd(const StepArgs &a, int src, int n0, hipStream_t s) {
+  const int grid = a.n_wg[EDGE ? 1 : 0];
+  if (grid == 0) return hipSuccess;
   if (a.medium_kind == kTable)
-    hipLaunchKernelGGL((step_kernel<T, C, kTable>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, a, src, n0);
+    hipLaunchKernelGGL((step_kernel<T, C, D, kTable, EDGE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, a, src, n0);
   else
-    hipLaunchKernelGGL((step_kernel<T, C, kDensity>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, a, src, n0);
+    hipLaunchKernelGGL((step_kernel<T, C, D, kDensity, EDGE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, a, src,
+                       n0);
   return hipGetLastError();
+}
+
+template <int C, int D, bool EDGE>
+hipError_t launch_cd(const StepArgs &a, int T, int src, int n0, hipStream_t s) {
+  switch (T) {
+    case 1: return launch_tcd<1, C, D, EDGE>(a, src, n0, s);
+    case 2: return launch_tcd<2, C, D, EDGE>(a, src, n0, s);
+    case 3: return launch_tcd<3, C, D, EDGE>(a, src, n0, s);
+    case 4: return launch_tcd<4, C, D, EDGE>(a, src, n0, s);
+    case 5: return launch_tcd<5, C, D, EDGE>(a, src, n0, s);
+    case 6: return launch_tcd<6, C, D, EDGE>(a, src, n0, s);
+    case 7: return launch_tcd<7, C, D, EDGE>(a, src, n0, s);
+    case 8: return launch_tcd<8, C, D, EDGE>(a, src, n0, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_step(const StepArgs &a, int T, int C, int src, int n0, hipStream_t stream) {
-  if (C != 1) return hipErrorInvalidValue;
-  switch (T) {
-    case 1: return launch_tc<1, 1>(a, src, n0, stream);
-    case 2: return launch_tc<2, 1>(a, src, n0, stream);
-    case 3: return launch_tc<3, 1>(a, src, n0, stream);
-    case 4: return launch_tc<4, 1>(a, src, n0, stream);
+hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream) {
+  // Edge work-groups always use the shallow prefetch (their state already
+  // fills the register file); interior ones take the requested depth.
+  if (cls != 0) {
+    switch (C) {
+      case 1: return launch_cd<1, 1, true>(a, T, src, n0, stream);
+      case 2: return launch_cd<2, 1, true>(a, T, src, n0, stream);
+      case 4: return launch_cd<4, 1, true>(a, T, src, n0, stream);
+    }
+    return hipErrorInvalidValue;
+  }
+  switch (C * 16 + D) {
+    case 1 * 16 + 1: return launch_cd<1, 1, false>(a, T, src, n0, stream);
+    case 1 * 16 + 2: return launch_cd<1, 2, false>(a, T, src, n0, stream);
+    case 1 * 16 + 4: return launch_cd<1, 4, false>(a, T, src, n0, stream);
+    case 2 * 16 + 1: return launch_cd<2, 1, false>(a, T, src, n0, stream);
+    case 2 * 16 + 2: return launch_cd<2, 2, false>(a, T, src, n0, stream);
+    case 4 * 16 + 1: return launch_cd<4, 1, false>(a, T, src, n0, stream);
   }
   return hipErrorInvalidValue;
 }

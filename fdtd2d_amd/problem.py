@@ -269,6 +269,9 @@ class Plan:
     def set_tblock(self, t: int) -> None:
         L.check(L.lib().fdtd2d_plan_set_tblock(self._h, t), "fdtd2d_plan_set_tblock")
 
+    def set_option(self, name: str, value: float) -> None:
+        L.check(L.lib().fdtd2d_plan_set_option(self._h, name.encode(), float(value)), "fdtd2d_plan_set_option")
+
     def info(self, name: str) -> float:
         v = C.c_double()
         L.check(L.lib().fdtd2d_plan_get_info(self._h, name.encode(), C.byref(v)), "fdtd2d_plan_get_info")

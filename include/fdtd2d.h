@@ -169,6 +169,12 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out);
 int fdtd2d_plan_set_stream(fdtd2d_plan *plan, void *stream);
 /* Time-blocking factor per launch (1..8; 0 = autotuned default). */
 int fdtd2d_plan_set_tblock(fdtd2d_plan *plan, int steps_per_launch);
+/* Launch-shape options: "tblock" (steps per launch, 1..8), "band" (rows per
+   tile, >= tblock; 0 = automatic), "band_cpml" (rows per tile of the tiles
+   that touch the CPML, 0 = automatic), "cols_per_lane" (1, 2 or 4), "prefetch"
+   (rows loaded ahead per wave: 1, 2 or 4 with 1 column per lane, 1 or 2 with
+   2, 1 with 4). Results are identical for every choice. */
+int fdtd2d_plan_set_option(fdtd2d_plan *plan, const char *name, double value);
 int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
 /* Advance nsteps steps (asynchronous on the plan's stream). */
 int fdtd2d_run(fdtd2d_plan *plan, int nsteps);

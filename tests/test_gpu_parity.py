@@ -29,37 +29,61 @@ def _compare_exact(gpu: dict, ref: dict, label: str):
                                  f"rel L2 {rel_l2(g, r):.3e}")
 
 
-@pytest.mark.parametrize("tblock", [1, 2, 3, 4])
-def test_plain_400_full_length(gpu_device, tblock):
+_REF400 = {}
+
+
+def _ref400():
+    if not _REF400:
+        setup = F.Setup.workload(1, seed=1)
+        ref = Oracle(setup, 32, threads=4)
+        ref.run(2000)
+        _REF400["setup"] = setup
+        _REF400["state"] = all_state(ref, 12)
+        r64 = Oracle(setup, 64, threads=4)
+        r64.run(2000)
+        _REF400["det64"] = r64.read_detectors()[0]
+    return _REF400
+
+
+@pytest.mark.parametrize("tblock,cols,depth", [(1, 1, 1), (2, 1, 2), (3, 1, 4), (4, 1, 1), (6, 1, 2), (8, 1, 4),
+                                               (1, 2, 1), (3, 2, 2), (4, 2, 1), (1, 4, 1), (2, 4, 1), (4, 4, 1)])
+def test_plain_400_full_length(gpu_device, tblock, cols, depth):
     """400x400, 2000 steps, CPML 12, linear plasma ramp with collisions."""
-    setup = F.Setup.workload(1, seed=1)
-    ref = Oracle(setup, 32, threads=4)
-    ref.run(2000)
-    rstate = all_state(ref, 12)
-    with F.Plan(setup, gpu_device, tblock=tblock) as plan:
-        print(f"T={tblock} band={plan.info('band')} tiles={plan.info('ntx')}x{plan.info('nty')}", flush=True)
+    R = _ref400()
+    setup, rstate = R["setup"], R["state"]
+    with F.Plan(setup, gpu_device) as plan:
+        plan.set_option("cols_per_lane", cols)
+        plan.set_option("tblock", tblock)
+        plan.set_option("prefetch", depth)
+        print(f"T={tblock} C={cols} D={depth} band={plan.info('band')} tiles={plan.info('ntx')}x{plan.info('nty')} "
+              f"cpml wgs={plan.info('workgroups_cpml')}/{plan.info('workgroups')}", flush=True)
         plan.run(2000)
         plan.sync()
         gstate = all_state(plan, 12)
-    _compare_exact(gstate, rstate, f"400^2 T={tblock}")
-    r64 = Oracle(setup, 64, threads=4)
-    r64.run(2000)
-    d64, _ = r64.read_detectors()
-    err = rel_l2(gstate["det"], d64)
-    print(f"T={tblock}: detector rel L2 vs FP64 oracle = {err:.3e}")
+    _compare_exact(gstate, rstate, f"400^2 T={tblock} C={cols}")
+    err = rel_l2(gstate["det"], R["det64"])
+    print(f"T={tblock} C={cols}: detector rel L2 vs FP64 oracle = {err:.3e}")
     assert err <= 1e-5
 
 
-@pytest.mark.parametrize("seed", list(range(12)))
+@pytest.mark.parametrize("seed", list(range(24)))
 def test_random_small_problems(gpu_device, seed):
     rng = np.random.default_rng(1000 + seed)
     p = random_problem(rng, nsteps=int(rng.integers(5, 40)))
-    tblock = int(rng.integers(1, 5))
-    print(f"seed {seed}: {p.nx}x{p.ny} P={p.npml} kind={p.medium_kind} T={tblock} "
-          f"nsrc={len(p.src_i)} ndet={len(p.det_i)}", flush=True)
+    cols = int(rng.choice([1, 2, 4]))
+    tblock = int(rng.integers(1, 9 if cols < 4 else 5))
+    band = int(rng.choice([0, tblock, 7, 16]))
+    depth = int(rng.choice({1: [1, 2, 4], 2: [1, 2], 4: [1]}[cols]))
+    print(f"seed {seed}: {p.nx}x{p.ny} P={p.npml} kind={p.medium_kind} T={tblock} C={cols} band={band} "
+          f"D={depth} nsrc={len(p.src_i)} ndet={len(p.det_i)}", flush=True)
     ref = Oracle(p, 32)
     ref.run(p.nsteps)
-    with F.Plan(p, gpu_device, tblock=tblock) as plan:
+    with F.Plan(p, gpu_device) as plan:
+        plan.set_option("cols_per_lane", cols)
+        plan.set_option("tblock", tblock)
+        plan.set_option("band", band)
+        plan.set_option("prefetch", depth)
+        plan.set_option("band_cpml", int(rng.choice([0, 1, 5, 9])))
         plan.run(p.nsteps)
         plan.sync()
         g = all_state(plan, p.npml)
