@@ -89,6 +89,8 @@ SIGNATURES = [
     ("fdtd2d_device_count", C.c_int, [C.POINTER(C.c_int)]),
     ("fdtd2d_setup_from_ini", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     ("fdtd2d_setup_ini_status", C.c_int, [C.c_void_p]),
+    ("fdtd2d_ini_lookup", C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_double,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     ("fdtd2d_setup_workload", C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("fdtd2d_setup_box", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
     ("fdtd2d_setup_problem", C.POINTER(ProblemC), [C.c_void_p]),
@@ -132,6 +134,14 @@ def lib() -> C.CDLL:
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def ini_lookup(text: str, section: str, name: str, kind: int = 1, default: float = 0.0):
+    """(value, parse_status) of one key, with the reference INIReader's rules."""
+    v, st = C.c_double(), C.c_int()
+    check(lib().fdtd2d_ini_lookup(text.encode("latin-1"), section.encode(), name.encode(), kind, default,
+                                  C.byref(v), C.byref(st)), "fdtd2d_ini_lookup")
+    return v.value, st.value
 
 
 def device_count() -> int:

@@ -802,6 +802,20 @@ int fdtd2d_setup_from_ini(const char *ini_path, fdtd2d_setup **out) {
 
 int fdtd2d_setup_ini_status(const fdtd2d_setup *s) { return s ? s->s.ini_status : 0; }
 
+int fdtd2d_ini_lookup(const char *text, const char *section, const char *name, int kind, double def,
+                      double *value, int *parse_status) {
+  if (!text || !section || !name || !value) return set_error(FDTD2D_EINVAL, "null argument");
+  fdtd2d::IniFile f = fdtd2d::IniFile::from_text(text);
+  if (parse_status) *parse_status = f.status();
+  switch (kind) {
+    case 0: *value = static_cast<double>(f.get_int(section, name, static_cast<long>(def))); break;
+    case 1: *value = f.get_real(section, name, def); break;
+    case 2: *value = f.get_bool(section, name, def != 0.0) ? 1.0 : 0.0; break;
+    default: return set_error(FDTD2D_EINVAL, "kind must be 0 (int), 1 (real) or 2 (bool)");
+  }
+  return FDTD2D_OK;
+}
+
 int fdtd2d_setup_workload(int workload, uint64_t seed, int shot, int steps_override,
                           fdtd2d_setup **out) {
   if (!out) return set_error(FDTD2D_EINVAL, "null out");

@@ -132,6 +132,13 @@ int fdtd2d_device_count(int *count);
 int fdtd2d_setup_from_ini(const char *ini_path, fdtd2d_setup **out);
 int fdtd2d_setup_ini_status(const fdtd2d_setup *s);
 
+/* One Settings.ini value from INI text in memory, read as the reference's
+ * INIReader does (F/INIlib/INIReader.cpp:30-60): kind 0 GetInteger
+ * (strtol base 0), 1 GetReal (strtod), 2 GetBoolean; `def` when absent or
+ * unparsable. *parse_status (may be NULL) gets 0 or the first bad line. */
+int fdtd2d_ini_lookup(const char *text, const char *section, const char *name, int kind, double def,
+                      double *value, int *parse_status);
+
 /* Synthetic workloads, seeded (std::mt19937_64):
  *   1  400x400 plain method (one point source, linear plasma ramp, 4 detectors)
  *   2  8192x8192 single grid (16-element steered array, parabolic plasma x (1+0.02 g))

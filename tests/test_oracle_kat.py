@@ -1,0 +1,348 @@
+"""Known-answer tests that pin the CPU oracle without running the reference.
+
+The reference ships no tests or golden vectors for its stepper, so each part
+of the oracle is pinned against something its own code does not restate:
+
+* interior stencil (Hx, Hy, Ez updates, index ranges, signs): a PEC-cavity
+  eigenmode evolves by the closed-form two-term recurrence of the Yee scheme;
+* Drude current (Jz from E^n, its coupling into Ez, the collision extension):
+  the same mode with a uniform plasma reduces to a 3-variable linear map, and
+  the measured oscillation frequency matches omega^2 = omega_p^2 + c^2 k^2;
+* leapfrog consistency: the discrete energy of a lossless PEC box (vacuum and
+  dielectric) is conserved to rounding;
+* CPML (psi updates, b/c tables, den with kappa): a pulse reflects below
+  -40 dB from a 12-cell CPML, and zeroing the psi coefficients (a planted
+  mistake in the oracle's inputs) makes the same test fail;
+* source / detector / component index rules: a frozen "marked" grid returns
+  the marker values, and the source accumulates its table exactly;
+* reciprocity of source and detector in an inhomogeneous PEC box;
+* FP32 vs FP64 agreement and bit-identical row-parallel threads.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import fdtd2d_amd as F
+from oracle.oracle import Oracle
+
+from helpers import CJ, EPS0, MU0, rel_l2
+
+C0 = 2.99792458e8
+
+
+def pec_box(nx, ny, steps, ds=1e-3):
+    """Vacuum PEC box (no CPML), DENSITY medium with zero density."""
+    return F.Setup.box(nx, ny, 0, steps, ds).problem()
+
+
+def mode(nx, ny, m, n):
+    i = np.arange(nx)[None, :]
+    j = np.arange(ny)[:, None]
+    return np.sin(m * np.pi * i / (nx - 1)) * np.sin(n * np.pi * j / (ny - 1))
+
+
+def mode_lambda(p, m, n):
+    dex, dhx = float(p.den_ex[1]), float(p.den_hx[1])
+    dey, dhy = float(p.den_ey[1]), float(p.den_hy[1])
+    sx = 2 * math.sin(m * math.pi / (2 * (p.nx - 1)))
+    sy = 2 * math.sin(n * math.pi / (2 * (p.ny - 1)))
+    return sx * sx * dex * dhx + sy * sy * dey * dhy
+
+
+def test_pec_eigenmode_closed_form():
+    nx, ny, N = 37, 29, 300
+    p = pec_box(nx, ny, N)
+    m, n = 3, 2
+    phi = mode(nx, ny, m, n)
+    g = float(np.float64(p.cb0)) * float(np.float64(p.db)) * mode_lambda(p, m, n)
+    theta = math.acos(1 - g / 2)
+    for prec, tol in ((64, 1e-10), (32, 2e-5)):
+        o = Oracle(p, prec)
+        o.write_field(F.EZ, phi)
+        for steps in (1, 7, N - 8):
+            o.run(steps)
+            k = o.steps_done
+            expect = math.cos(k * theta) - (g / 2) / math.sin(theta) * math.sin(k * theta)
+            ez = o.read_field(F.EZ)
+            assert rel_l2(ez, expect * phi) < tol, (prec, k)
+
+
+def test_eigenmode_distinguishes_wrong_operator():
+    """A mistake in the curl (here: the mode of a box one cell larger) moves
+    the frequency, so the closed form is sensitive to index errors."""
+    nx, ny = 37, 29
+    p = pec_box(nx, ny, 200)
+    phi = mode(nx, ny, 3, 2)
+    o = Oracle(p, 64)
+    o.write_field(F.EZ, phi)
+    o.run(200)
+    q = pec_box(nx + 1, ny, 200)
+    g_wrong = float(np.float64(q.cb0)) * float(np.float64(q.db)) * mode_lambda(q, 3, 2)
+    th = math.acos(1 - g_wrong / 2)
+    wrong = math.cos(200 * th) - (g_wrong / 2) / math.sin(th) * math.sin(200 * th)
+    assert rel_l2(o.read_field(F.EZ), wrong * phi) > 1e-3
+
+
+def drude_modal(p, g, ne, N):
+    """3-variable map of one cavity mode with uniform plasma: e (Ez/phi),
+    u (accumulated H image) and j (Jz/phi)."""
+    cb, db = float(np.float64(p.cb0)), float(np.float64(p.db))
+    lam = g / (cb * db)
+    cj, d, kdt = float(np.float32(p.cj)), float(np.float32(p.jz_decay)), p.jz_dt
+    ne = float(np.float32(ne))
+    e, u, jj = 1.0, 0.0, 0.0
+    out = [e]
+    for _ in range(N):
+        u = u + e
+        jj = d * jj + ((cj * e) * ne) * kdt
+        e = e + cb * (-db * lam * u - jj)
+        out.append(e)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("nu", [0.0, 3e9])
+def test_drude_mode_matches_modal_recurrence(nu):
+    nx, ny, N = 33, 33, 400
+    p = pec_box(nx, ny, N)
+    dt = p.dt
+    f = 4e9
+    ne = EPS0 * (2 * math.pi * f) ** 2 / CJ
+    p.density = np.full((ny, nx), ne, np.float32)
+    p.jz_decay = (1 - nu * dt / 2) / (1 + nu * dt / 2)
+    p.jz_dt = dt / (1 + nu * dt / 2)
+    phi = mode(nx, ny, 2, 1)
+    g = float(np.float64(p.cb0)) * float(np.float64(p.db)) * mode_lambda(p, 2, 1)
+    expect = drude_modal(p, g, ne, N)
+    o = Oracle(p, 64)
+    o.write_field(F.EZ, phi)
+    for k in (1, 50, N):
+        o.run(k - o.steps_done)
+        assert rel_l2(o.read_field(F.EZ), expect[k] * phi) < 1e-9, k
+        # Jz carries the same mode shape
+        jz = o.read_field(F.JZ)
+        c = np.sum(jz * phi) / np.sum(phi * phi)
+        assert rel_l2(jz, c * phi) < 1e-9
+
+
+def test_plasma_dispersion_physics():
+    """Oscillation frequency of a cavity mode filled with cold plasma:
+    omega^2 = omega_p^2 + c^2 k^2 (collisionless), to well within 1%."""
+    nx, ny, N = 41, 41, 6000
+    ds = 2e-3
+    p = pec_box(nx, ny, N, ds)
+    fp = 2e9
+    ne = EPS0 * (2 * math.pi * fp) ** 2 / CJ
+    p.density = np.full((ny, nx), ne, np.float32)
+    phi = mode(nx, ny, 1, 1)
+    o = Oracle(p, 64)
+    o.write_field(F.EZ, phi)
+    o.run(N)
+    # Ez(t) at the mode centre from the component trace is not recorded, so
+    # rerun with a detector there
+    p.det_i = np.array([nx // 2], np.int32)
+    p.det_j = np.array([ny // 2], np.int32)
+    o = Oracle(p, 64)
+    o.write_field(F.EZ, phi)
+    o.run(N)
+    tr = o.read_detectors()[0][:, 0]
+    tr = tr[1:]
+    s = np.sign(tr)
+    cross = np.nonzero(s[1:] * s[:-1] < 0)[0]
+    t0, t1 = cross[0], cross[-1]
+    periods = (len(cross) - 1) / 2
+    f_meas = periods / ((t1 - t0) * p.dt)
+    k2 = (math.pi / ((nx - 1) * ds)) ** 2 * 2
+    f_theory = math.sqrt((2 * math.pi * fp) ** 2 + C0 ** 2 * k2) / (2 * math.pi)
+    assert abs(f_meas / f_theory - 1) < 5e-3
+
+
+def energy(o, cb_cells, db):
+    ez = o.read_field(F.EZ)
+    hx0, hy0 = o.read_field(F.HX), o.read_field(F.HY)
+    o.run(1)
+    hx1, hy1 = o.read_field(F.HX), o.read_field(F.HY)
+    return float(np.sum(ez * ez / cb_cells) + (np.sum(hx0 * hx1) + np.sum(hy0 * hy1)) / db)
+
+
+def test_energy_conserved_in_lossless_pec_box():
+    nx, ny, N = 48, 40, 120
+    p = pec_box(nx, ny, N)
+    # lossless dielectric blob through a material table: CA = 1, CB = dt/eps
+    dt = p.dt
+    eps = np.array([1.0, 1.0, 4.0, 2.2]) * EPS0
+    p.medium_kind = F.MEDIUM_TABLE
+    p.ca = np.ones(4, np.float32)
+    p.cb = (dt / eps).astype(np.float32)
+    p.ne = np.zeros(4, np.float32)
+    p.pec_material = -1
+    mat = np.zeros((ny, nx), np.int32)
+    mat[10:25, 12:30] = 2
+    mat[28:35, 5:15] = 3
+    p.material = mat
+    p.density = None
+    rng = np.random.default_rng(3)
+    o = Oracle(p, 64)
+    ez0 = rng.standard_normal((ny, nx))
+    ez0[0, :] = ez0[-1, :] = 0
+    ez0[:, 0] = ez0[:, -1] = 0
+    o.write_field(F.EZ, ez0)
+    cb_cells = p.cb.astype(np.float64)[mat]
+    db = float(np.float64(p.db))
+    inner = (slice(1, ny - 1), slice(1, nx - 1))
+    cbi = np.ones_like(cb_cells)
+    cbi[inner] = cb_cells[inner]
+    w = []
+    for _ in range(6):
+        w.append(energy(o, cbi, db))
+        o.run(15)
+    w = np.array(w)
+    assert np.max(np.abs(w - w[0])) / w[0] < 1e-11
+
+
+def cpml_problem(n, steps, src_off=0, det_off=20, P=12, kill_psi=False):
+    s = F.Setup.workload(1, seed=0, steps=steps)  # CPML grading of the workloads
+    base = s.problem()
+    box = F.Setup.box(n, n, P, steps, 1e-3).problem()
+    # rebuild the CPML tables for this size with the workload parameters
+    box.density = np.zeros((n, n), np.float32)
+    box.be_x, box.ce_x, box.bh_x, box.ch_x = base.be_x, base.ce_x, base.bh_x, base.ch_x
+    box.be_y, box.ce_y, box.bh_y, box.ch_y = base.be_y, base.ce_y, base.bh_y, base.ch_y
+    if kill_psi:
+        z = np.zeros(P, np.float32)
+        box.ce_x = [z, z]
+        box.ce_y = [z, z]
+        box.ch_x = [z[:-1], z[:-1]]
+        box.ch_y = [z[:-1], z[:-1]]
+    c = n // 2
+    box.src_i = np.array([c + src_off], np.int32)
+    box.src_j = np.array([c], np.int32)
+    t = np.arange(steps + 1) * box.dt
+    tau, t0 = 40 * box.dt, 160 * box.dt
+    box.src_table = (-np.exp(-((t - t0) / tau) ** 2))[:, None].astype(np.float32)
+    box.det_i = np.array([c + det_off], np.int32)
+    box.det_j = np.array([c], np.int32)
+    return box
+
+
+def test_cpml_reflection_below_40db_and_planted_mistake_fails():
+    small, steps = 120, 420
+    # reference: large box, boundary echoes arrive after `steps`
+    big = cpml_problem(1000, steps)
+    ref = Oracle(big, 64, threads=8)
+    ref.run(steps)
+    r = ref.read_detectors()[0][:, 0]
+    dut = Oracle(cpml_problem(small, steps), 64, threads=8)
+    dut.run(steps)
+    d = dut.read_detectors()[0][:, 0]
+    refl = np.max(np.abs(d - r)) / np.max(np.abs(r))
+    assert refl < 1e-2, f"CPML reflection {20 * np.log10(refl):.1f} dB"
+    bad = Oracle(cpml_problem(small, steps, kill_psi=True), 64, threads=8)
+    bad.run(steps)
+    b = bad.read_detectors()[0][:, 0]
+    assert np.max(np.abs(b - r)) / np.max(np.abs(r)) > 5e-2
+
+
+def test_marked_grid_detector_component_and_source_indexing():
+    nx, ny, N = 23, 17, 6
+    p = pec_box(nx, ny, N)
+    p.cb0 = 0.0  # frozen fields: Ez keeps its marker, only sources change it
+    p.db = 0.0
+    p.det_i = np.array([0, 5, 22, 7], np.int32)
+    p.det_j = np.array([0, 9, 16, 3], np.int32)
+    p.comp_i = np.array([4, 21], np.int32)
+    p.comp_j = np.array([2, 15], np.int32)
+    p.src_i = np.array([7, 7], np.int32)
+    p.src_j = np.array([3, 3], np.int32)
+    p.src_table = np.zeros((N + 1, 2), np.float32)
+    p.src_table[:, 0] = np.arange(N + 1) * 0.5
+    p.src_table[:, 1] = 0.25
+    o = Oracle(p, 32)
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    o.write_field(F.EZ, jj * 1000.0 + ii)
+    o.write_field(F.HX, (jj * 1000.0 + ii + 0.25)[:-1, :])
+    o.write_field(F.HY, (jj * 1000.0 + ii + 0.5)[:, :-1])
+    o.run(N)
+    det, comp = o.read_detectors()
+    for n in range(1, N + 1):
+        marks = p.det_j * 1000.0 + p.det_i
+        injected = sum(p.src_table[k, 0] + p.src_table[k, 1] for k in range(1, n + 1))
+        marks[3] -= injected  # detector 3 sits on the doubled source cell (7, 3)
+        assert np.allclose(det[n], marks, rtol=0, atol=1e-3)
+        assert np.array_equal(comp[1][n], p.comp_j * 1000.0 + p.comp_i + 0.25)
+        assert np.array_equal(comp[2][n], p.comp_j * 1000.0 + p.comp_i + 0.5)
+    assert np.all(det[0] == 0)
+
+
+def test_pec_material_forces_ez_to_zero():
+    nx, ny = 20, 20
+    p = pec_box(nx, ny, 3)
+    dt = p.dt
+    p.medium_kind = F.MEDIUM_TABLE
+    p.ca = np.ones(2, np.float32)
+    p.cb = np.full(2, dt / EPS0, np.float32)
+    p.ne = np.zeros(2, np.float32)
+    p.pec_material = 1
+    mat = np.zeros((ny, nx), np.int32)
+    mat[5:8, 6:9] = 1
+    p.material = mat
+    p.density = None
+    o = Oracle(p, 32)
+    o.write_field(F.EZ, np.ones((ny, nx)))
+    o.run(1)
+    ez = o.read_field(F.EZ)
+    assert np.all(ez[5:8, 6:9] == 0)
+    assert np.all(ez[10:12, 10:12] != 0)
+
+
+def test_reciprocity_in_inhomogeneous_box():
+    nx, ny, N = 50, 44, 300
+    p = pec_box(nx, ny, N)
+    dt = p.dt
+    eps = np.array([1.0, 1.0, 3.0]) * EPS0
+    p.medium_kind = F.MEDIUM_TABLE
+    p.ca = np.ones(3, np.float32)
+    p.cb = (dt / eps).astype(np.float32)
+    p.ne = np.zeros(3, np.float32)
+    p.pec_material = -1
+    mat = np.zeros((ny, nx), np.int32)
+    mat[15:30, 20:35] = 2
+    p.material = mat
+    p.density = None
+    t = np.arange(N + 1) * dt
+    pulse = (-np.exp(-((t - 30 * dt) / (8 * dt)) ** 2))[:, None].astype(np.float32)
+
+    def trace(src, det):
+        q = p
+        q.src_i, q.src_j = np.array([src[0]], np.int32), np.array([src[1]], np.int32)
+        q.det_i, q.det_j = np.array([det[0]], np.int32), np.array([det[1]], np.int32)
+        q.src_table = pulse
+        o = Oracle(q, 64)
+        o.run(N)
+        return o.read_detectors()[0][:, 0]
+
+    a, b = (8, 10), (40, 35)
+    ab, ba = trace(a, b), trace(b, a)
+    assert np.max(np.abs(ab)) > 0
+    assert rel_l2(ab, ba) < 1e-10
+
+
+def test_fp32_close_to_fp64_on_plain_workload():
+    s = F.Setup.workload(1, seed=1, steps=1000)
+    o32, o64 = Oracle(s, 32, threads=4), Oracle(s, 64, threads=4)
+    o32.run(1000)
+    o64.run(1000)
+    assert rel_l2(o32.read_detectors()[0], o64.read_detectors()[0]) < 1e-5
+    assert rel_l2(o32.read_field(F.EZ), o64.read_field(F.EZ)) < 1e-4
+
+
+def test_threads_are_bit_identical():
+    s = F.Setup.workload(1, seed=1, steps=60)
+    a, b = Oracle(s, 32, threads=1), Oracle(s, 32, threads=8)
+    a.run(60)
+    b.run(60)
+    for w in (F.EZ, F.HX, F.HY, F.JZ):
+        assert np.array_equal(a.read_field(w), b.read_field(w))
