@@ -16,6 +16,16 @@ Prints ONE JSON line (rank 0). Besides the metric it carries
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--tblock T]
                   [--cpu-steps S] [--no-cpu] [--traffic-file F]
+
+Profiling helpers (no bench line):
+  --copy-probe MIB      copy a MIB-MiB buffer 5 times with torch (the plain
+                        copy kernel that calibrates rocprofv3's byte counters)
+  --pmc-summary FETCH_CSV WRITE_CSV COPY_FETCH_CSV COPY_WRITE_CSV
+                        turn four `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE`
+                        counter_collection.csv files (bench run, copy probe)
+                        into the traffic file and a text summary on stdout
+  --stats-summary KERNEL_STATS_CSV
+                        print a `rocprofv3 --stats` kernel table as text
 """
 
 from __future__ import annotations
@@ -38,8 +48,99 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_probe(mib: int) -> int:
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {mib} MiB x 5")
+    return 0
+
+
+def _read_counter(path: str, counter: str) -> dict:
+    """Mean counter value (KiB per dispatch) per kernel name."""
+    import csv
+
+    acc: dict = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            v = acc.setdefault(row["Kernel_Name"], [0.0, 0])
+            v[0] += float(row["Counter_Value"])
+            v[1] += 1
+    return {k: v[0] / v[1] for k, v in acc.items()}
+
+
+def _pick(d: dict, pred) -> tuple:
+    names = [k for k in d if pred(k)]
+    if not names:
+        raise SystemExit("no matching kernel in counter file")
+    k = max(names, key=lambda n: d[n])
+    return k, d[k]
+
+
+def pmc_summary(paths: list, copy_mib: int, traffic_file: str) -> int:
+    fetch = _read_counter(paths[0], "FETCH_SIZE")
+    write = _read_counter(paths[1], "WRITE_SIZE")
+    cfetch = _read_counter(paths[2], "FETCH_SIZE")
+    cwrite = _read_counter(paths[3], "WRITE_SIZE")
+    interior = lambda n: "step_kernel" in n and "false" in n
+    kname, fkib = _pick(fetch, interior)
+    _, wkib = _pick(write, interior)
+    cname, cf = _pick(cfetch, lambda n: "copy" in n.lower() or "elementwise" in n.lower())
+    _, cw = _pick(cwrite, lambda n: "copy" in n.lower() or "elementwise" in n.lower())
+    copy_bytes = copy_mib * (1 << 20)
+    fscale = copy_bytes / (cf * 1024.0)
+    wscale = copy_bytes / (cw * 1024.0)
+    fb = fkib * 1024.0 * fscale
+    wb = wkib * 1024.0 * wscale
+    rec = {
+        "kernel": kname,
+        "fetch_kib_raw": round(fkib, 1),
+        "write_kib_raw": round(wkib, 1),
+        "copy_kernel": cname,
+        "copy_bytes": copy_bytes,
+        "copy_fetch_kib_raw": round(cf, 1),
+        "copy_write_kib_raw": round(cw, 1),
+        "fetch_scale": round(fscale, 4),
+        "write_scale": round(wscale, 4),
+        "fetch_bytes_per_launch": round(fb),
+        "write_bytes_per_launch": round(wb),
+        "bytes_per_launch": round(fb + wb),
+    }
+    Path(traffic_file).write_text(json.dumps(rec, indent=1) + "\n")
+    print(f"kernel: {kname}")
+    print(f"FETCH_SIZE {fkib:.1f} KiB/dispatch x {fscale:.3f} = {fb / 1e6:.2f} MB")
+    print(f"WRITE_SIZE {wkib:.1f} KiB/dispatch x {wscale:.3f} = {wb / 1e6:.2f} MB")
+    print(f"HBM traffic per launch: {(fb + wb) / 1e6:.2f} MB")
+    print(f"calibration: {cname} copying {copy_bytes} B reads FETCH {cf:.1f} KiB, WRITE {cw:.1f} KiB")
+    return 0
+
+
+def stats_summary(path: str) -> int:
+    import csv
+
+    with open(path, newline="") as f:
+        rows = list(csv.DictReader(f))
+    print(f"{'calls':>7} {'avg_us':>10} {'min_us':>10} {'max_us':>10} {'pct':>6}  kernel")
+    for r in rows:
+        print(f"{r['Calls']:>7} {float(r['AverageNs']) / 1e3:>10.1f} {float(r['MinNs']) / 1e3:>10.1f} "
+              f"{float(r['MaxNs']) / 1e3:>10.1f} {float(r['Percentage']):>6.2f}  {r['Name']}")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--pmc-summary", nargs=4, default=None)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--stats-summary", default=None)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
@@ -49,6 +150,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_8192.json"))
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
+    if args.pmc_summary:
+        return pmc_summary(args.pmc_summary, args.copy_mib, args.traffic_file)
+    if args.stats_summary:
+        return stats_summary(args.stats_summary)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -78,24 +185,29 @@ def main() -> int:
     plan.sync()
     elapsed = time.perf_counter() - t_start
     avg_ms, nlaunch = plan.kernel_ms()
+    cells_int = plan.info("cells_interior")
+    plan_c = plan.info("cols_per_lane")
     det, _ = plan.read_detectors()
     plan.close()
     ok = bool((det == det).all())  # no NaN in the traces
     cells = nx * ny
     value = cells * args.steps / elapsed / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
-    # launches of the timed region: all T steps except possibly the last
-    full_launch_updates = cells * T
+    # the timed kernel is the interior one: T steps over the interior cells
+    full_launch_updates = cells_int * T
     achieved = BYTES_PER_CELL_UPDATE * full_launch_updates / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
     tf = Path(args.traffic_file)
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            rec = json.loads(tf.read_text())
+            # only a record taken on this launch shape describes this kernel
+            if f"step_kernel<{T}, {int(plan_c)}, " in rec.get("kernel", ""):
+                traffic = rec.get("bytes_per_launch")
         except Exception:
             traffic = None
     log(f"GPU: {value:.1f} Gcell/s, {ms_per_step:.3f} ms/step, kernel avg {avg_ms:.3f} ms over {nlaunch} "
-        f"launches of T={T}, finite traces: {ok}")
+        f"launches of T={T} ({cells_int / cells:.3f} of the grid), finite traces: {ok}")
 
     cpu = None
     if not args.no_cpu:
@@ -144,7 +256,9 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel_avg_ms": round(avg_ms, 4),
-            "bytes_per_launch": BYTES_PER_CELL_UPDATE * full_launch_updates,
+            "kernel": "interior step kernel",
+            "cells_per_launch": int(cells_int),
+            "bytes_per_launch": int(BYTES_PER_CELL_UPDATE * full_launch_updates),
         },
         "cpu_baseline": cpu,
     }

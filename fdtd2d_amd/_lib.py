@@ -112,6 +112,9 @@ SIGNATURES = [
     ("fdtd2d_read_psi", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
     ("fdtd2d_write_psi", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
     ("fdtd2d_plan_footprint", C.c_int, [C.POINTER(ProblemC), C.POINTER(C.c_uint64)]),
+    ("fdtd2d_tile_lists", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                                    C.POINTER(C.c_int), C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
     ("fdtd2d_plan_destroy", C.c_int, [C.c_void_p]),
     ("fdtd2d_plan_time_kernels", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_plan_kernel_ms", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
@@ -142,6 +145,27 @@ def ini_lookup(text: str, section: str, name: str, kind: int = 1, default: float
     check(lib().fdtd2d_ini_lookup(text.encode("latin-1"), section.encode(), name.encode(), kind, default,
                                   C.byref(v), C.byref(st)), "fdtd2d_ini_lookup")
     return v.value, st.value
+
+
+def tile_lists(nx: int, ny: int, npml: int, tblock: int = 0, cols_per_lane: int = 1, band: int = 0,
+               band_cpml: int = 0, events=()):
+    """(entries, geometry): the tile partition a plan of this shape launches.
+
+    events are (i, j) cells of sources, detectors and probes. entries is a
+    list of (tile, first row, end row, class), geometry the dict
+    {vw, halo, band, band_cpml}. Host-only; needs no GPU."""
+    n = C.c_size_t()
+    g = (C.c_int * 4)()
+    L = lib()
+    ne = len(events)
+    ei = (C.c_int * max(ne, 1))(*[e[0] for e in events])
+    ej = (C.c_int * max(ne, 1))(*[e[1] for e in events])
+    args = (nx, ny, npml, tblock, cols_per_lane, band, band_cpml, ei, ej, ne)
+    check(L.fdtd2d_tile_lists(*args, None, 0, C.byref(n), g), "fdtd2d_tile_lists")
+    buf = (C.c_int * (4 * max(n.value, 1)))()
+    check(L.fdtd2d_tile_lists(*args, buf, n.value, C.byref(n), g), "fdtd2d_tile_lists")
+    ent = [tuple(buf[4 * k:4 * k + 4]) for k in range(n.value)]
+    return ent, {"vw": g[0], "halo": g[1], "band": g[2], "band_cpml": g[3]}
 
 
 def device_count() -> int:

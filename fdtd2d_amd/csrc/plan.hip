@@ -34,6 +34,10 @@ constexpr size_t kAlign = 256;
 // lanes/rows they do not own there, so their loop issues the same memory
 // operations every row (nothing reads these rows).
 constexpr int kTrashRows = 64;
+// Discarded stores (halo lanes, rows outside the band) go to this many
+// 64*C-float slots: few enough to stay resident in L2, so they cost no HBM
+// writes, and spread over enough lines to avoid one hot L2 channel.
+constexpr int kTrashSlots = 64;
 // Steps per launch when none is set: on MI355X the interior kernel is issue
 // bound near 8 levels and memory bound below (8192^2: 81 Gcell/s at 1,
 // ~300 at 4, ~520 at 8 with 1 column per lane and 128-row bands).
@@ -129,8 +133,11 @@ struct fdtd2d_plan {
   int T = 0;  // 0: default
   int C = 1;
   int band = 0;  // 0: automatic
-  int D = 1;     // rows of prefetch per wave (interior kernel)
+  int D = 2;     // rows of prefetch per wave (interior kernel); the row loop is unrolled by D
   int band_cpml = 0;               // rows per CPML-class work-group (0: automatic)
+  double cells_class[2] = {0, 0};  // grid cells owned by interior / CPML-class tiles
+  int hc = 0;                      // rows per CPML-class sub-band in use
+  int n_nojz = 0;                  // interior tiles skipping the Jz update
   int4 *wg_buf = nullptr;          // device copy of the two work-group lists
   hipStream_t aux = nullptr;       // CPML-class launches run here, beside the interior ones
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -141,6 +148,17 @@ struct fdtd2d_plan {
   double sum_ms = 0;
   int count = 0;
   int launches_T[fdtd2d::kMaxT + 1] = {0};
+  // Host facts the tiling needs. Prefix counts of the rows / columns whose
+  // den values differ from the uniform interior ones (the interior kernel
+  // takes those as scalars), and per row the column span [lo, hi] holding
+  // plasma (Ne > 0) or a nonzero Jz: windows outside every span skip the Jz
+  // update (there Jz stays +0 and Ez - Jz = Ez).
+  std::vector<int> srow_pref, scol_pref;
+  std::vector<int> ne_lo, ne_hi, jz_lo, jz_hi;
+  // cells holding a source, detector or probe, sorted by (row, column): the
+  // tiles whose window holds one take the CPML-class kernel, so the interior
+  // kernel carries no event code
+  std::vector<int2> ev_cells;
 };
 
 namespace {
@@ -165,33 +183,141 @@ int pick_band(int nx, int ny, int VW, int T) {
   return std::max(H, T);
 }
 
-int set_tiling(fdtd2d_plan *pl) {
-  const int T = pl->T > 0 ? pl->T : kDefaultT;
-  const int halo = ((T + pl->C - 1) / pl->C) * pl->C;
-  const int VW = 64 * pl->C - 2 * halo;
-  fdtd2d::StepArgs &a = pl->a;
-  a.halo = halo;
-  a.vw = VW;
-  a.ntx = (pl->L.nx + VW - 1) / VW;
-  a.nwgx = (a.ntx + fdtd2d::kWavesPerWG - 1) / fdtd2d::kWavesPerWG;
-  a.H = pl->band > 0 ? std::max(pl->band, T) : pick_band(pl->L.nx, pl->L.ny, VW, T);
-  a.nty = (pl->L.ny + a.H - 1) / a.H;
-  // work-group lists: interior / CPML, each in (column group, band) order;
-  // CPML-class bands are cut into sub-bands of hc rows.
-  const int hc = pl->band_cpml > 0 ? std::max(pl->band_cpml, T) : std::max(16, T);
-  std::vector<int4> lists[2];
-  for (int wgx = 0; wgx < a.nwgx; ++wgx)
-    for (int ty = 0; ty < a.nty; ++ty) {
-      const int jb0 = ty * a.H, jb1 = std::min(pl->L.ny, jb0 + a.H);
-      if (!fdtd2d::wg_touches_cpml(a, T, wgx, jb0, jb1)) {
-        lists[0].push_back(make_int4(wgx, jb0, jb1, 0));
+// Tile lists of one launch shape, one entry {tx, r0, r1, 0} per wave:
+// [0] interior, [1] CPML class. A (tile, band) whose loads touch a CPML strip
+// or the outer ring is cut into sub-bands of hc rows; the sub-bands that still
+// touch go to the CPML list, runs of the others are merged back into one
+// interior entry. Host-only, so the partition can be checked without a GPU.
+struct Tiling {
+  int T = 0, halo = 0, vw = 0, ntx = 0, H = 0, nty = 0, hc = 0;
+  std::vector<int4> lists[2];  // padded to whole work-groups
+  int n_tiles[2] = {0, 0};
+  int n_nojz = 0;  // interior tiles that skip the Jz update
+  double cells[2] = {0, 0};
+};
+
+struct TileMasks {
+  const int *srow_pref = nullptr, *scol_pref = nullptr;  // [ny+1], [nx+1]
+  const int *ne_lo = nullptr, *ne_hi = nullptr, *jz_lo = nullptr, *jz_hi = nullptr;  // [ny]
+  const int2 *ev = nullptr;  // event cells {column, row}, sorted by (row, column)
+  size_t nev = 0;
+};
+
+Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml, const TileMasks *m = nullptr) {
+  Tiling t;
+  t.T = T;
+  t.halo = ((T + C - 1) / C) * C;
+  t.vw = 64 * C - 2 * t.halo;
+  t.ntx = (nx + t.vw - 1) / t.vw;
+  t.H = band > 0 ? std::max(band, T) : pick_band(nx, ny, t.vw, T);
+  t.nty = (ny + t.H - 1) / t.H;
+  t.hc = band_cpml > 0 ? std::max(band_cpml, T) : std::max(32, T);
+  // window of rows / columns a tile loads
+  auto window = [&](int tx, int r0, int r1, int &jl0, int &jl1, int &cs, int &ce) {
+    jl0 = std::max(0, r0 - T);
+    jl1 = std::min(ny, r1 + T);
+    cs = std::max(0, tx * t.vw - t.halo);
+    ce = std::min(nx, tx * t.vw + t.vw + t.halo);
+  };
+  auto touches = [&](int tx, int r0, int r1) {
+    if (fdtd2d::tile_touches_cpml(nx, ny, P, t.vw, t.halo, T, tx, r0, r1)) return true;
+    if (!m) return false;
+    int jl0, jl1, cs, ce;
+    window(tx, r0, r1, jl0, jl1, cs, ce);
+    if (m->srow_pref && (m->srow_pref[jl1] != m->srow_pref[jl0] || m->scol_pref[ce] != m->scol_pref[cs]))
+      return true;
+    // first event at or after row jl0
+    size_t lo = 0, hi = m->nev;
+    while (lo < hi) {
+      const size_t mid = (lo + hi) / 2;
+      if (m->ev[mid].y < jl0) lo = mid + 1;
+      else hi = mid;
+    }
+    for (size_t q = lo; q < m->nev && m->ev[q].y < jl1; ++q)
+      if (m->ev[q].x >= cs && m->ev[q].x < ce) return true;
+    return false;
+  };
+  auto needs_jz = [&](const int4 &w) {
+    if (!m || !m->ne_lo) return 1;
+    int jl0, jl1, cs, ce;
+    window(w.x, w.y, w.z, jl0, jl1, cs, ce);
+    for (int r = jl0; r < jl1; ++r) {
+      if (m->ne_lo[r] <= m->ne_hi[r] && m->ne_lo[r] < ce && m->ne_hi[r] >= cs) return 1;
+      if (m->jz_lo[r] <= m->jz_hi[r] && m->jz_lo[r] < ce && m->jz_hi[r] >= cs) return 1;
+    }
+    return 0;
+  };
+  std::vector<int4> *lists = t.lists;
+  for (int tx = 0; tx < t.ntx; ++tx)
+    for (int ty = 0; ty < t.nty; ++ty) {
+      const int jb0 = ty * t.H, jb1 = std::min(ny, jb0 + t.H);
+      if (!touches(tx, jb0, jb1)) {
+        lists[0].push_back(make_int4(tx, jb0, jb1, 0));
         continue;
       }
-      for (int r0 = jb0; r0 < jb1; r0 += hc) {
-        const int r1 = std::min(jb1, r0 + hc);
-        lists[fdtd2d::wg_touches_cpml(a, T, wgx, r0, r1) ? 1 : 0].push_back(make_int4(wgx, r0, r1, 0));
+      int run0 = -1;  // start of the current run of interior sub-bands
+      for (int r0 = jb0; r0 < jb1; r0 += t.hc) {
+        const int r1 = std::min(jb1, r0 + t.hc);
+        if (touches(tx, r0, r1)) {
+          if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, r0, 0));
+          run0 = -1;
+          lists[1].push_back(make_int4(tx, r0, r1, 0));
+        } else if (run0 < 0) {
+          run0 = r0;
+        }
       }
+      if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, jb1, 0));
     }
+  // Order by column group of 4 tiles, then first row, then tile: the 4 waves
+  // of a work-group get neighbouring tiles and consecutive work-groups go down
+  // the same columns. Pad each list to whole work-groups with empty entries.
+  for (int c = 0; c < 2; ++c) {
+    std::stable_sort(lists[c].begin(), lists[c].end(), [](const int4 &u, const int4 &v) {
+      const int gu = u.x / fdtd2d::kWavesPerWG, gv = v.x / fdtd2d::kWavesPerWG;
+      if (gu != gv) return gu < gv;
+      if (u.y != v.y) return u.y < v.y;
+      return u.x < v.x;
+    });
+    for (int4 &w : lists[c]) {
+      w.w = c == 0 ? needs_jz(w) : 1;
+      const long long c0 = static_cast<long long>(w.x) * t.vw;
+      const long long c1 = std::min<long long>(nx, c0 + t.vw);
+      t.cells[c] += static_cast<double>(std::max<long long>(0, c1 - c0)) * (w.z - w.y);
+    }
+    t.n_tiles[c] = static_cast<int>(lists[c].size());
+    if (c == 0)
+      for (const int4 &w : lists[c]) t.n_nojz += w.w == 0;
+    while (lists[c].size() % fdtd2d::kWavesPerWG) lists[c].push_back(make_int4(0, 0, 0, 0));
+  }
+  return t;
+}
+
+int set_tiling(fdtd2d_plan *pl) {
+  const int T = pl->T > 0 ? pl->T : kDefaultT;
+  TileMasks m;
+  m.srow_pref = pl->srow_pref.data();
+  m.scol_pref = pl->scol_pref.data();
+  m.ne_lo = pl->ne_lo.data();
+  m.ne_hi = pl->ne_hi.data();
+  m.jz_lo = pl->jz_lo.data();
+  m.jz_hi = pl->jz_hi.data();
+  m.ev = pl->ev_cells.data();
+  m.nev = pl->ev_cells.size();
+  Tiling t = make_tiling(pl->L.nx, pl->L.ny, pl->a.P, T, pl->C, pl->band, pl->band_cpml, &m);
+  fdtd2d::StepArgs &a = pl->a;
+  a.halo = t.halo;
+  a.vw = t.vw;
+  a.trash_slots = std::min(kTrashSlots, kTrashRows * pl->L.pitch / (64 * pl->C));
+  a.ntx = t.ntx;
+  a.H = t.H;
+  a.nty = t.nty;
+  pl->hc = t.hc;
+  for (int c = 0; c < 2; ++c) {
+    pl->cells_class[c] = t.cells[c];
+    a.n_tiles[c] = t.n_tiles[c];
+  }
+  pl->n_nojz = t.n_nojz;
+  const std::vector<int4> *lists = t.lists;
   const size_t n = lists[0].size() + lists[1].size();
   if (pl->wg_buf) {
     hipFree(pl->wg_buf);
@@ -204,10 +330,10 @@ int set_tiling(fdtd2d_plan *pl) {
   HIP_TRY(hipMemcpy(pl->wg_buf + lists[0].size(), lists[1].data(), lists[1].size() * sizeof(int4),
                     hipMemcpyHostToDevice),
           "hipMemcpy lists");
-  a.wg_list[0] = pl->wg_buf;
-  a.wg_list[1] = pl->wg_buf + lists[0].size();
-  a.n_wg[0] = static_cast<int>(lists[0].size());
-  a.n_wg[1] = static_cast<int>(lists[1].size());
+  a.tile_list[0] = pl->wg_buf;
+  a.tile_list[1] = pl->wg_buf + lists[0].size();
+  a.n_wg[0] = static_cast<int>(lists[0].size()) / fdtd2d::kWavesPerWG;
+  a.n_wg[1] = static_cast<int>(lists[1].size()) / fdtd2d::kWavesPerWG;
   return FDTD2D_OK;
 }
 
@@ -256,6 +382,46 @@ int fdtd2d_device_count(int *count) {
   if (e == hipErrorNoDevice) return FDTD2D_OK;
   HIP_TRY(e, "hipGetDeviceCount");
   *count = n;
+  return FDTD2D_OK;
+}
+
+int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, int band, int band_cpml,
+                      const int *ev_i, const int *ev_j, int nev, int *entries, size_t capacity, size_t *count,
+                      int *geometry) {
+  if (!count) return set_error(FDTD2D_EINVAL, "null count");
+  if (nx < 3 || ny < 3 || npml < 0 || 2 * npml > nx - 2 || 2 * npml > ny - 2)
+    return set_error(FDTD2D_EINVAL, "grid or CPML size out of range");
+  const int T = tblock > 0 ? tblock : kDefaultT;
+  if (T > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "tblock must be 1..8");
+  if (cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4)
+    return set_error(FDTD2D_EINVAL, "cols_per_lane must be 1, 2 or 4");
+  if (band < 0 || band_cpml < 0) return set_error(FDTD2D_EINVAL, "negative band");
+  if (nev < 0 || (nev > 0 && (!ev_i || !ev_j))) return set_error(FDTD2D_EINVAL, "bad event cells");
+  std::vector<int2> ev;
+  for (int e = 0; e < nev; ++e) ev.push_back(make_int2(ev_i[e], ev_j[e]));
+  std::sort(ev.begin(), ev.end(), [](const int2 &u, const int2 &v) { return u.y != v.y ? u.y < v.y : u.x < v.x; });
+  TileMasks m;
+  m.ev = ev.data();
+  m.nev = ev.size();
+  const Tiling t = make_tiling(nx, ny, npml, T, cols_per_lane, band, band_cpml, &m);
+  *count = static_cast<size_t>(t.n_tiles[0]) + t.n_tiles[1];
+  if (geometry) {
+    geometry[0] = t.vw;
+    geometry[1] = t.halo;
+    geometry[2] = t.H;
+    geometry[3] = t.hc;
+  }
+  if (!entries) return FDTD2D_OK;
+  if (capacity < *count) return set_error(FDTD2D_EINVAL, "entries too small");
+  size_t k = 0;
+  for (int c = 0; c < 2; ++c)
+    for (int q = 0; q < t.n_tiles[c]; ++q, ++k) {
+      const int4 &w = t.lists[c][q];
+      entries[4 * k] = w.x;
+      entries[4 * k + 1] = w.y;
+      entries[4 * k + 2] = w.z;
+      entries[4 * k + 3] = c;
+    }
   return FDTD2D_OK;
 }
 
@@ -405,6 +571,47 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
     if ((rc = upload(pl, L.off_row[6], hxy_s.data(), ny * 4))) return fail(rc);
     if ((rc = upload(pl, L.off_row[7], ezy_s.data(), ny * 4))) return fail(rc);
   }
+  // uniform interior den values (bitwise) and the rows / columns that differ
+  float u_hx, u_ex, u_hy, u_ey;
+  {
+    const int im = (nx - 1) / 2, jm = (ny - 1) / 2;
+    u_hx = p->den_hx[im];
+    u_ex = p->den_ex[im];
+    u_hy = p->den_hy[jm];
+    u_ey = p->den_ey[jm];
+    auto same = [](float x, float y) { return std::memcmp(&x, &y, 4) == 0; };
+    pl->scol_pref.assign(nx + 1, 0);
+    for (int i = 0; i < nx; ++i) {
+      const bool sp = i >= nx - 1 || !same(p->den_hx[i], u_hx) || !same(p->den_ex[i], u_ex);
+      pl->scol_pref[i + 1] = pl->scol_pref[i] + sp;
+    }
+    pl->srow_pref.assign(ny + 1, 0);
+    for (int j = 0; j < ny; ++j) {
+      const bool sp = j >= ny - 1 || !same(p->den_hy[j], u_hy) || !same(p->den_ey[j], u_ey);
+      pl->srow_pref[j + 1] = pl->srow_pref[j] + sp;
+    }
+  }
+  {
+    for (int e = 0; e < p->nsrc; ++e) pl->ev_cells.push_back(make_int2(p->src_i[e], p->src_j[e]));
+    for (int e = 0; e < p->ndet; ++e) pl->ev_cells.push_back(make_int2(p->det_i[e], p->det_j[e]));
+    for (int e = 0; e < p->ncomp; ++e) pl->ev_cells.push_back(make_int2(p->comp_i[e], p->comp_j[e]));
+    std::sort(pl->ev_cells.begin(), pl->ev_cells.end(),
+              [](const int2 &u, const int2 &v) { return u.y != v.y ? u.y < v.y : u.x < v.x; });
+  }
+  // column spans of plasma per row; Jz starts at +0 everywhere
+  pl->ne_lo.assign(ny, INT32_MAX);
+  pl->ne_hi.assign(ny, -1);
+  pl->jz_lo.assign(ny, INT32_MAX);
+  pl->jz_hi.assign(ny, -1);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t c = static_cast<size_t>(j) * nx + i;
+      const float ne = p->medium_kind == FDTD2D_MEDIUM_TABLE ? p->ne[p->material[c]] : p->density[c];
+      if (ne > 0.f) {
+        pl->ne_lo[j] = std::min(pl->ne_lo[j], i);
+        pl->ne_hi[j] = std::max(pl->ne_hi[j], i);
+      }
+    }
   if ((rc = upload(pl, L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
   {
     std::vector<int4> ri(ny);
@@ -413,7 +620,7 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
       int dh, de;
       std::memcpy(&dh, &p->den_hy[jc], 4);
       std::memcpy(&de, &p->den_ey[jc], 4);
-      ri[j] = make_int4(dh, de, ev_off[j], ev_off[j + 1]);
+      ri[j] = make_int4(dh, de, ev_off[j] < ev_off[j + 1] ? ev_off[j] : -1, ev_off[j + 1]);
     }
     if ((rc = upload(pl, L.off_rowinfo, ri.data(), ri.size() * sizeof(int4)))) return fail(rc);
   }
@@ -440,6 +647,10 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   a.dxf = p->dx_f;
   a.dyf = p->dy_f;
   a.jz_dt = p->jz_dt;
+  a.uden_hx = u_hx;
+  a.uden_ex = u_ex;
+  a.uden_hy = u_hy;
+  a.uden_ey = u_ey;
   for (int s = 0; s < 2; ++s) {
     a.ez[s] = reinterpret_cast<float *>(pl->base + L.off_fields[s][0]);
     a.hx[s] = reinterpret_cast<float *>(pl->base + L.off_fields[s][1]);
@@ -532,10 +743,16 @@ int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value)
   if (n == "pitch") *value = pl->L.pitch;
   else if (n == "tblock") *value = current_T(pl);
   else if (n == "band") *value = pl->a.H;
+  else if (n == "band_cpml") *value = pl->hc;
   else if (n == "ntx") *value = pl->a.ntx;
   else if (n == "nty") *value = pl->a.nty;
   else if (n == "workgroups") *value = pl->a.n_wg[0] + pl->a.n_wg[1];
   else if (n == "workgroups_cpml") *value = pl->a.n_wg[1];
+  else if (n == "tiles_interior") *value = pl->a.n_tiles[0];
+  else if (n == "tiles_cpml") *value = pl->a.n_tiles[1];
+  else if (n == "tiles_nojz") *value = pl->n_nojz;
+  else if (n == "cells_interior") *value = pl->cells_class[0];
+  else if (n == "cells_cpml") *value = pl->cells_class[1];
   else if (n == "tile_cols") *value = pl->a.vw;
   else if (n == "bytes") *value = static_cast<double>(pl->L.total);
   else if (n == "steps") *value = pl->n;
@@ -558,7 +775,6 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       e0 = get_event(pl);
       e1 = get_event(pl);
       if (!e0 || !e1) return set_error(FDTD2D_EDEVICE, "hipEventCreate");
-      HIP_TRY(hipEventRecord(e0, pl->stream), "hipEventRecord");
     }
     // CPML work-groups on the aux stream, interior ones on the main stream;
     // both read buffer set src and write the other, so they may overlap.
@@ -569,13 +785,15 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       if (e != hipSuccess) return hip_fail(e, "CPML step kernel launch");
       HIP_TRY(hipEventRecord(pl->ev_join, pl->aux), "hipEventRecord join");
     }
+    // timing events bracket the interior kernel alone on the main stream
+    if (pl->timing) HIP_TRY(hipEventRecord(e0, pl->stream), "hipEventRecord");
     hipError_t e = fdtd2d::launch_step(pl->a, 0, t, pl->C, pl->D, pl->src, pl->n, pl->stream);
     if (e != hipSuccess) return hip_fail(e, "step kernel launch");
-    if (pl->a.n_wg[1] > 0) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
     if (pl->timing) {
       HIP_TRY(hipEventRecord(e1, pl->stream), "hipEventRecord");
       pl->pending.push_back({e0, e1});
     }
+    if (pl->a.n_wg[1] > 0) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
     pl->launches_T[t] += 1;
     pl->src = 1 - pl->src;
     pl->n += t;
@@ -650,6 +868,23 @@ int fdtd2d_write_field(fdtd2d_plan *pl, int which, const float *in, size_t count
   for (int s = 0; s < 2; ++s)
     HIP_TRY(hipMemcpy2D(bufs[s], pl->L.pitch * 4, in, cols * 4, cols * 4, rows, hipMemcpyHostToDevice),
             "hipMemcpy2D H2D field");
+  if (which == FDTD2D_JZ) {
+    // tiles whose window now holds a nonzero Jz (any bits but +0) must update it
+    for (int j = 0; j < rows; ++j) {
+      int lo = INT32_MAX, hi = -1;
+      for (int i = 0; i < cols; ++i) {
+        uint32_t b;
+        std::memcpy(&b, &in[static_cast<size_t>(j) * cols + i], 4);
+        if (b != 0u) {
+          lo = std::min(lo, i);
+          hi = std::max(hi, i);
+        }
+      }
+      pl->jz_lo[j] = lo;
+      pl->jz_hi[j] = hi;
+    }
+    return set_tiling(pl);
+  }
   return FDTD2D_OK;
 }
 

@@ -26,6 +26,7 @@ struct Event {
 struct StepArgs {
   int nx, ny, pitch;   // pitch: floats per row (multiple of 64), all fields
   int trash_rows;      // scratch rows below row ny-1 of every field plane
+  int trash_slots;     // 64*C-float slots of those rows the waves spread their discarded stores over
   int P;               // CPML thickness (0 = none)
   int xs;              // floats per row of the x-strip psi arrays (2*P rounded up)
   int nsteps;
@@ -47,7 +48,8 @@ struct StepArgs {
   const int32_t *hyx_slot, *ezx_slot;          // [pitch], -1 outside the strips
   const float *hxy_b, *hxy_c, *ezy_b, *ezy_c;  // [ny]
   const int32_t *hxy_slot, *ezy_slot;          // [ny]
-  // per row: {den_hy bits, den_ey bits, first event, end event}, [ny]
+  // per row: {den_hy bits, den_ey bits, first event or -1 when the row has
+  // none, end event}, [ny]
   const int4 *rowinfo;
   // events by row
   const int32_t *ev_off;  // [ny+1]
@@ -59,18 +61,24 @@ struct StepArgs {
   float *comp;
   int ncomp;
   long long comp_plane;
+  // den values of every row / column an interior tile loads (bitwise equal
+  // there; rows and columns with other values make a tile CPML class)
+  float uden_hx, uden_ex, uden_hy, uden_ey;
   // tiling (fixed per plan; launches of fewer steps reuse the same tiles)
   int vw;        // stored columns per tile = 64*C - 2*halo
   int halo;      // recomputed columns on each side, >= steps per launch
   int H;         // rows per band
   int ntx, nty;  // tiles across / down
-  int nwgx;      // work-groups across (4 tiles each)
-  // Work-group lists, entry = {column group, first row, end row, 0}:
-  // [0] interior work-groups (no tile loads a CPML cell) on bands of H rows,
-  // [1] the rest, on shorter bands so that they finish well inside the
+  // Tile lists, one entry per wave = {tile, first row, end row, flags}; wave
+  // w of work-group b takes entry 4b+w, entries with no rows are padding.
+  // flags bit 0: the tile's window holds plasma or a nonzero Jz (interior
+  // tiles without it skip the Jz update, load and store).
+  // [0] interior tiles (no load touches a CPML cell), on bands of up to H
+  // rows; [1] the rest, on sub-bands short enough to finish well inside the
   // interior launch they run beside. Each list is one launch.
-  const int4 *wg_list[2];
-  int n_wg[2];
+  const int4 *tile_list[2];
+  int n_tiles[2];  // entries in use
+  int n_wg[2];     // work-groups launched (entries / 4, rounded up)
 };
 
 // One launch advances T steps (1..kMaxT).
@@ -85,19 +93,16 @@ hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src,
 // loads a cell of a CPML strip.
 // (With no CPML the outer PEC ring counts as the strip: the interior kernel
 // applies no index masks, so it must never load a ring cell.)
-inline bool wg_touches_cpml(const StepArgs &a, int T, int wgx, int jb0, int jb1) {
-  const int P = a.P > 0 ? a.P : 1;
-  const int halo = a.halo;
-  const int VW = a.vw;
-  const int jl0 = jb0 - T > 0 ? jb0 - T : 0, jl1 = jb1 + T < a.ny ? jb1 + T : a.ny;
-  if (jl0 <= P || jl1 >= a.ny - P - 1) return true;
-  for (int w = 0; w < kWavesPerWG; ++w) {
-    const int tx = wgx * kWavesPerWG + w;
-    if (tx >= a.ntx) break;
-    const int cs = tx * VW - halo, ce = cs + VW + 2 * halo;
-    if (cs <= P || ce >= a.nx - P - 1) return true;
-  }
-  return false;
+// True when tile tx (vw stored columns, halo recomputed columns each side)
+// over output rows [jb0, jb1) loads a cell of a CPML strip or of the outer
+// ring (with no CPML the ring counts as a strip of width 1): such a tile
+// takes the CPML-class kernel.
+inline bool tile_touches_cpml(int nx, int ny, int P, int vw, int halo, int T, int tx, int jb0, int jb1) {
+  const int Pe = P > 0 ? P : 1;
+  const int jl0 = jb0 - T > 0 ? jb0 - T : 0, jl1 = jb1 + T < ny ? jb1 + T : ny;
+  if (jl0 <= Pe || jl1 >= ny - Pe - 1) return true;
+  const int cs = tx * vw - halo, ce = cs + vw + 2 * halo;
+  return cs <= Pe || ce >= nx - Pe - 1;
 }
 
 }  // namespace fdtd2d

@@ -14,12 +14,14 @@
 //  * all state (Ez, Hx, Hy, Jz, the psi strips) is ping-ponged between two
 //    buffer sets so that halo recomputation in neighbouring tiles reads the
 //    step-n0 values while the owner writes step n0+T;
-//  * two kernels share a launch: EDGE work-groups (any tile loads a cell of a
-//    CPML strip or of the outer ring) carry the psi state and the reference's
-//    index ranges; interior work-groups carry neither, so their per-cell code
-//    is straight-line (no lane-divergent branches), and they address HBM
-//    with one 32-bit row offset shared by all streams - the interior loop is
-//    otherwise limited by scalar-unit issue.
+//  * two kernels share a launch: EDGE tiles (the tile loads a cell of a CPML
+//    strip or of the outer ring, a row or column with non-uniform den, or a
+//    source / detector / probe cell) carry the psi state, the per-row tables,
+//    the events and the reference's index ranges; interior tiles carry none
+//    of these, so their per-cell code is straight-line (no lane-divergent
+//    branches, no per-row scalars), and they address HBM with one 32-bit row
+//    offset shared by all streams. Interior tiles whose window holds no
+//    plasma and no Jz skip the Jz update and its loads and stores.
 // Per cell and step the arithmetic is that of the reference kernels
 // (F/kernel.cu:127-537, abbreviated as in fdtd2d.h), in the same order and
 // without FMA contraction, so results match the CPU oracle bit for bit:
@@ -52,6 +54,16 @@ __device__ __forceinline__ int cld(const int *p, long long i) { return ((cint_pt
 __device__ __forceinline__ float cld(const float *p, long long i) { return ((cfloat_ptr)p)[i]; }
 __device__ __forceinline__ int cld_i(const void *p, long long i) { return ((cint_ptr)p)[i]; }
 
+// x-neighbours across lanes with wavefront-wide DPP shifts (GFX9 wave_shl:1
+// and wave_shr:1): lane l receives lane l+1 / l-1. The end lanes receive 0;
+// they lie in the tile's halo, whose columns the levels give up one by one.
+__device__ __forceinline__ float lane_from_next(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_from_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
 template <int C, bool EDGE>
 struct Level {
   float ez[C], hx[C], hy[C], jz[C];
@@ -70,7 +82,7 @@ __device__ __forceinline__ float jz_update(float jz, float decay, float cj, floa
 template <int C, bool EDGE>
 struct Cols {
   int c0;
-  float den_ex[C], den_hx[C];
+  float den_ex[EDGE ? C : 1], den_hx[EDGE ? C : 1];  // interior: uniform scalars
   float hyx_b[EDGE ? C : 1], hyx_c[EDGE ? C : 1], ezx_b[EDGE ? C : 1], ezx_c[EDGE ? C : 1];
   int hyx_s[EDGE ? C : 1], ezx_s[EDGE ? C : 1];
 };
@@ -88,16 +100,23 @@ struct RowS {
 template <bool EDGE>
 __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
   RowS<EDGE> q;
+  q.hxy_s = q.ezy_s = -1;
+  q.hxy_b = q.hxy_c = q.ezy_b = q.ezy_c = 0.f;
+  if constexpr (!EDGE) {
+    // interior tiles: uniform den, and no event lies in their window
+    q.den_hy = a.uden_hy;
+    q.den_ey = a.uden_ey;
+    q.e0 = q.e1 = 0;
+    return q;
+  }
   const bool in = r >= 0 && r < a.ny;
   const long long ri = 4LL * (in ? r : 0);
   q.den_hy = __int_as_float(cld_i(a.rowinfo, ri));
   q.den_ey = __int_as_float(cld_i(a.rowinfo, ri + 1));
   const int e0 = cld_i(a.rowinfo, ri + 2), e1 = cld_i(a.rowinfo, ri + 3);
-  q.e0 = in ? e0 : 0;
-  q.e1 = in ? e1 : 0;
-  q.hxy_s = q.ezy_s = -1;
-  q.hxy_b = q.hxy_c = q.ezy_b = q.ezy_c = 0.f;
-  if constexpr (EDGE) {
+  q.e0 = in && e0 >= 0 ? e0 : 0;
+  q.e1 = in && e0 >= 0 ? e1 : 0;
+  {
     if (r >= 0 && r <= a.ny - 2) {
       q.hxy_s = cld(a.hxy_slot, r);
       q.ezy_s = cld(a.ezy_slot, r);
@@ -112,7 +131,7 @@ __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
 
 // One level: level t-1's row r (p), Ez^{t-1}(r+1) (ez_s) and Hx^{t-1/2}(r-1)
 // (hx_n) -> level t's row r (o).
-template <int C, bool EDGE, int MED>
+template <int C, bool EDGE, int MED, bool JZ, bool EV>
 __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl,
                                         const RowS<EDGE> &rs, const Level<C, EDGE> &p, const float (&ez_s)[C],
                                         const float (&hx_n)[C], int r, int step, int own0, int own1,
@@ -126,7 +145,7 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
   float ez_e[C];
 #pragma unroll
   for (int c = 0; c < C - 1; ++c) ez_e[c] = p.ez[c + 1];
-  ez_e[C - 1] = __shfl_down(p.ez[0], 1);
+  ez_e[C - 1] = lane_from_next(p.ez[0]);
   // ---- H (F/kernel.cu:167-196, 198-240, 288-318, 321-362)
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -135,7 +154,10 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
     const float dEy = p.ez[c] - ez_s[c];
     const float dEx = ez_e[c] - p.ez[c];
     float hx = p.hx[c] + a.db * (dEy * rs.den_hy);
-    float hy = p.hy[c] + a.db * (dEx * cl.den_hx[c]);
+    float den_hx;
+    if constexpr (EDGE) den_hx = cl.den_hx[c];
+    else den_hx = a.uden_hx;
+    float hy = p.hy[c] + a.db * (dEx * den_hx);
     if constexpr (EDGE) {
       o.phxy[c] = p.phxy[c];
       o.phyx[c] = p.phyx[c];
@@ -159,7 +181,7 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
   float hy_w[C];
 #pragma unroll
   for (int c = 1; c < C; ++c) hy_w[c] = o.hy[c - 1];
-  hy_w[0] = __shfl_up(o.hy[C - 1], 1);
+  hy_w[0] = lane_from_prev(o.hy[C - 1]);
   // ---- Jz (F/kernel.cu:127-163) and Ez (F/kernel.cu:413-537)
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -178,11 +200,23 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
       cb = a.cb0;
       ne = __uint_as_float(p.med[c]);
     }
-    const float jz_new = jz_update(p.jz[c], a.jz_decay, a.cj, p.ez[c], ne, a.jz_dt);
-    const float jz = (updE && ne > 0.f) ? jz_new : p.jz[c];
     const float dHy = o.hy[c] - hy_w[c];
     const float dHx = hx_n[c] - o.hx[c];
-    float ez = ca * p.ez[c] + cb * ((dHy * cl.den_ex[c] + dHx * rs.den_ey) - jz);
+    float den_ex;
+    if constexpr (EDGE) den_ex = cl.den_ex[c];
+    else den_ex = a.uden_ex;
+    const float curl = dHy * den_ex + dHx * rs.den_ey;
+    float jz = 0.f, ez;
+    if constexpr (JZ) {
+      // Jz^{n+1/2} from Ez^n where Ne > 0; (...) - Jz as the reference orders it
+      const float jz_new = jz_update(p.jz[c], a.jz_decay, a.cj, p.ez[c], ne, a.jz_dt);
+      jz = (updE && ne > 0.f) ? jz_new : p.jz[c];
+      ez = ca * p.ez[c] + cb * (curl - jz);
+    } else {
+      // the window holds no plasma and Jz = +0, so curl - Jz = curl exactly
+      (void)ne;
+      ez = ca * p.ez[c] + cb * curl;
+    }
     if constexpr (MED == kTable) ez = pec ? 0.f : ez;
     if constexpr (EDGE) {
       ez = updE ? ez : p.ez[c];
@@ -203,6 +237,7 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
     o.ez[c] = ez;
     o.med[c] = p.med[c];
   }
+  if constexpr (!EV) return;
   // ---- soft sources, then detector samples of this step (F/kernel.cu:621-805)
   for (int e = rs.e0; e < rs.e1; ++e) {
     Event ev;
@@ -313,7 +348,7 @@ __device__ __forceinline__ void store_row(float *__restrict__ base, long long ro
 
 // Level 0 of row k: the step-n0 state (zeros for rows outside [0, ny) or
 // past the band's last loaded row).
-template <int C, bool EDGE, int MED>
+template <int C, bool EDGE, int MED, bool JZ>
 __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm, const Cols<C, EDGE> &cl,
                                             int src, int k, bool rv, bool lane_in, Level<C, EDGE> &in) {
   const int pitch = a.pitch;
@@ -329,10 +364,20 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
     oload<C>(sm.ez, off, in.ez);
     oload<C>(sm.hx, off, in.hx);
     oload<C>(sm.hy, off, in.hy);
-    oload<C>(sm.jz, off, in.jz);
-    oload<C>(sm.med, off, mv);
+    if constexpr (JZ) {
+      oload<C>(sm.jz, off, in.jz);
+    } else {
 #pragma unroll
-    for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
+      for (int c = 0; c < C; ++c) in.jz[c] = 0.f;
+    }
+    if constexpr (JZ || MED == kTable) {
+      oload<C>(sm.med, off, mv);
+#pragma unroll
+      for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) in.med[c] = 0u;
+    }
   } else {
     const bool ld = rv && lane_in;
     load_row<C>(a.ez[src], k, cl.c0, pitch, ld, in.ez);
@@ -366,9 +411,33 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
   }
 }
 
+// Levels 1..T of one row iteration: level t computes row k-t from level t-1.
+// EV: apply the sources and take the samples of the events on those rows
+// (only CPML-class tiles hold event cells).
+template <int T, int C, bool EDGE, int MED, bool JZ, bool EV>
+__device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl, int n0,
+                                       int k, int jb0, int jb1, int own0, int own1, Level<C, EDGE> prev,
+                                       Level<C, EDGE> (&st)[T + 1], const RowS<EDGE> (&rs)[T + 1]) {
+  float nxt_ez[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) nxt_ez[c] = st[0].ez[c];
+#pragma unroll
+  for (int t = 1; t <= T; ++t) {
+    const int r = k - t;
+    Level<C, EDGE> out;
+    const bool own_row = r >= jb0 && r < jb1;
+    advance<C, EDGE, MED, JZ, EV>(a, lut_s, cl, rs[t], prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row,
+                                  out);
+    prev = st[t];
+    st[t] = out;
+#pragma unroll
+    for (int c = 0; c < C; ++c) nxt_ez[c] = out.ez[c];
+  }
+}
+
 // One row iteration: level 0 takes row k, level t computes row k-t, and
 // row k-T is stored.
-template <int T, int C, bool EDGE, int MED>
+template <int T, int C, bool EDGE, int MED, bool JZ>
 __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, const float4 *lut_s,
                                          const Cols<C, EDGE> &cl, int src, int n0, int k, int jb0, int jb1,
                                          int own0, int own1, bool lane_owns, unsigned trash_off,
@@ -379,21 +448,8 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
   rs[0] = row_scalars<EDGE>(a, k);
   Level<C, EDGE> prev = st[0];
   st[0] = in;
-  float nxt_ez[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) nxt_ez[c] = in.ez[c];
   // ---- levels 1..T, each one row behind the previous
-#pragma unroll
-  for (int t = 1; t <= T; ++t) {
-    const int r = k - t;
-    Level<C, EDGE> out;
-    const bool own_row = r >= jb0 && r < jb1;
-    advance<C, EDGE, MED>(a, lut_s, cl, rs[t], prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
-    prev = st[t];
-    st[t] = out;
-#pragma unroll
-    for (int c = 0; c < C; ++c) nxt_ez[c] = out.ez[c];
-  }
+  levels<T, C, EDGE, MED, JZ, EDGE>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st, rs);
   // ---- store step n0+T for owned cells of row k-T
   const int r = k - T;
   if constexpr (!EDGE) {
@@ -407,7 +463,7 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
     ostore<C>(dst.ez, off, o.ez);
     ostore<C>(dst.hx, off, o.hx);
     ostore<C>(dst.hy, off, o.hy);
-    ostore<C>(dst.jz, off, o.jz);
+    if constexpr (JZ) ostore<C>(dst.jz, off, o.jz);
   } else if (r >= jb0 && r < jb1) {
     const int pitch = a.pitch;
     const Level<C, EDGE> &o = st[T];
@@ -435,7 +491,7 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
 // D = rows fetched ahead of the one being consumed (memory-level
 // parallelism per wave); the row loop is unrolled by D so the prefetch ring
 // is indexed statically.
-template <int T, int C, int D, bool EDGE, int MED>
+template <int T, int C, int D, bool EDGE, int MED, bool JZ>
 __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s, int src, int n0,
                                          int tx, int jb0, int jb1) {
   const int HALO = a.halo;  // >= T, a multiple of C
@@ -453,9 +509,9 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
   for (int c = 0; c < C; ++c) {
     const int col = cl.c0 + c;
     const bool in = col >= 0 && col < pitch;
-    cl.den_ex[c] = in ? a.den_ex[col] : 0.f;
-    cl.den_hx[c] = in ? a.den_hx[col] : 0.f;
     if constexpr (EDGE) {
+      cl.den_ex[c] = in ? a.den_ex[col] : 0.f;
+      cl.den_hx[c] = in ? a.den_hx[col] : 0.f;
       cl.hyx_s[c] = in ? a.hyx_slot[col] : -1;
       cl.ezx_s[c] = in ? a.ezx_slot[col] : -1;
       cl.hyx_b[c] = in ? a.hyx_b[col] : 0.f;
@@ -488,7 +544,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
     }
   }
   // Scratch slot of this wave in the rows below the grid (see row_step).
-  const unsigned nslot = static_cast<unsigned>(a.trash_rows * pitch / (64 * C));
+  const unsigned nslot = static_cast<unsigned>(a.trash_slots);
   const unsigned slot = (blockIdx.x * kWavesPerWG + (threadIdx.x >> 6)) % nslot;
   const unsigned trash_off =
       (static_cast<unsigned>(ny) * static_cast<unsigned>(pitch) + slot * 64u * C + static_cast<unsigned>(lane * C)) * 4u;
@@ -499,7 +555,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     const int kk = EDGE ? jl0 + u : min(jl0 + u, jl1 - 1);
-    load_level0<C, EDGE, MED>(a, sm, cl, src, kk, jl0 + u < jl1, lane_in, pre[u]);
+    load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, jl0 + u < jl1, lane_in, pre[u]);
   }
   // Iterations k in [jl0, jl1 + T), rounded up to a multiple of D: the extra
   // rows lie below the band's output rows and store nothing.
@@ -509,8 +565,8 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       const Level<C, EDGE> in = pre[u];
       const int kn = k + u + D;
       const int kk = EDGE ? kn : min(kn, jl1 - 1);
-      load_level0<C, EDGE, MED>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
-      row_step<T, C, EDGE, MED>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns, trash_off, in,
+      load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
+      row_step<T, C, EDGE, MED, JZ>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns, trash_off, in,
                                 st, rs);
     }
   }
@@ -525,15 +581,20 @@ __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a
   }
   // Launch index -> list position: consecutive positions go to one XCD
   // (work-groups are dealt to the 8 XCDs round robin), and the lists are
-  // ordered by column group then band, so an XCD works down vertically
+  // ordered by column group then first row, so an XCD works down vertically
   // adjacent bands whose shared halo rows stay in its L2.
   const int G = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, q = G >> 3, rr = G & 7;
   const int pos = xcd * q + min(xcd, rr) + (b >> 3);
-  const int4 w = a.wg_list[EDGE ? 1 : 0][pos];
-  const int tx = w.x * kWavesPerWG + (threadIdx.x >> 6);
-  if (tx >= a.ntx) return;
-  run_tile<T, C, D, EDGE, MED>(a, lut_s, src, n0, tx, w.y, w.z);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long e = 4LL * (pos * kWavesPerWG + wave);
+  const int tx = cld_i(a.tile_list[EDGE ? 1 : 0], e), r0 = cld_i(a.tile_list[EDGE ? 1 : 0], e + 1),
+            r1 = cld_i(a.tile_list[EDGE ? 1 : 0], e + 2), flags = cld_i(a.tile_list[EDGE ? 1 : 0], e + 3);
+  if (r0 >= r1) return;  // padding entry
+  if (EDGE || (flags & 1))
+    run_tile<T, C, D, EDGE, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+  else
+    run_tile<T, C, D, EDGE, MED, false>(a, lut_s, src, n0, tx, r0, r1);
 }
 
 template <int T, int C, int D, bool EDGE>

@@ -182,6 +182,9 @@ int fdtd2d_plan_set_tblock(fdtd2d_plan *plan, int steps_per_launch);
    (rows loaded ahead per wave: 1, 2 or 4 with 1 column per lane, 1 or 2 with
    2, 1 with 4). Results are identical for every choice. */
 int fdtd2d_plan_set_option(fdtd2d_plan *plan, const char *name, double value);
+/* Plan facts by name: "pitch", "tblock", "band", "band_cpml", "ntx", "nty",
+   "workgroups", "workgroups_cpml", "tile_cols", "bytes", "steps",
+   "cols_per_lane", "prefetch", "cells_interior", "cells_cpml". */
 int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
 /* Advance nsteps steps (asynchronous on the plan's stream). */
 int fdtd2d_run(fdtd2d_plan *plan, int nsteps);
@@ -196,12 +199,27 @@ int fdtd2d_write_field(fdtd2d_plan *plan, int which, const float *in, size_t cou
  * HYX [2][ny][npml-1], EZX [2][ny][npml], EZY [2][npml][nx]. */
 int fdtd2d_read_psi(fdtd2d_plan *plan, int which, float *out, size_t count);
 int fdtd2d_write_psi(fdtd2d_plan *plan, int which, const float *in, size_t count);
+/* The tile partition a plan of this shape launches, computed on the host (no
+   device needed), for uniform den tables and the nev event cells (sources,
+   detectors, probes) at (ev_i[e], ev_j[e]): entries[4k..4k+3] = {tile, first
+   row, end row, class} for k < *count, class 0 interior kernel, 1 CPML-class
+   kernel; tile t stores columns [t*vw, t*vw+vw). geometry (4 ints, may be
+   null) = {vw, halo columns, rows per band, rows per CPML sub-band}. entries
+   may be null to query *count. tblock 0 = the default. No reference
+   counterpart: the reference launches one thread per cell
+   (F/kernel.cu:2027-2061). */
+int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, int band, int band_cpml,
+                      const int *ev_i, const int *ev_j, int nev, int *entries, size_t capacity, size_t *count,
+                      int *geometry);
 /* Device bytes this plan holds (the footprint admission control counts). */
 int fdtd2d_plan_footprint(const fdtd2d_problem *p, uint64_t *bytes);
 int fdtd2d_plan_destroy(fdtd2d_plan *plan);
 
-/* Average duration (ms) of the last `count` step-kernel launches, timed with
- * HIP events on the plan's stream; 0 disables timing. */
+/* Average duration (ms) of the interior step-kernel launches (the kernel that
+ * updates the cells of the interior work-groups, `cells_interior` of
+ * fdtd2d_plan_get_info, for up to `tblock` steps per launch) since timing was
+ * enabled, from HIP events recorded around each launch on the plan's stream;
+ * 0 disables timing. */
 int fdtd2d_plan_time_kernels(fdtd2d_plan *plan, int enable);
 int fdtd2d_plan_kernel_ms(fdtd2d_plan *plan, double *avg_ms, int *count);
 

@@ -120,3 +120,72 @@ def test_write_field_initial_state(gpu_device):
         plan.sync()
         g = all_state(plan, 5)
     _compare_exact(g, all_state(ref, 5), "initial state")
+
+
+def _run_pair(p, gpu_device, opts, label, prepare=None):
+    ref = Oracle(p, 32, threads=4)
+    with F.Plan(p, gpu_device) as plan:
+        for k, v in opts.items():
+            plan.set_option(k, v)
+        if prepare:
+            prepare(ref, plan)
+        info = {k: plan.info(k) for k in ("tiles_interior", "tiles_cpml", "tiles_nojz")}
+        print(f"{label}: {p.nx}x{p.ny} P={p.npml} {opts} {info}", flush=True)
+        ref.run(p.nsteps)
+        plan.run(p.nsteps)
+        plan.sync()
+        g = all_state(plan, p.npml)
+    _compare_exact(g, all_state(ref, p.npml), label)
+    return info
+
+
+@pytest.mark.parametrize("seed", list(range(10)))
+def test_interior_paths(gpu_device, seed):
+    """Grids large enough for interior tiles, with and without plasma in
+    their windows, both media, every launch shape."""
+    rng = np.random.default_rng(5000 + seed)
+    P = int(rng.choice([0, 5, 11, 16]))
+    cols = int(rng.choice([1, 2, 4]))
+    nx = int(rng.integers(260, 640)) if cols < 4 else int(rng.integers(760, 900))
+    p = random_problem(rng, nx=nx, ny=int(rng.integers(200, 520)), npml=P, nsteps=int(rng.integers(9, 30)),
+                       table=bool(seed % 2))
+    tblock = int(rng.integers(1, 9 if cols < 4 else 5))
+    opts = {"cols_per_lane": cols, "tblock": tblock, "band": int(rng.choice([0, 16, 64])),
+            "prefetch": int(rng.choice({1: [1, 2, 4], 2: [1, 2], 4: [1]}[cols])),
+            "band_cpml": int(rng.choice([0, 8, 32]))}
+    info = _run_pair(p, gpu_device, opts, f"interior seed {seed}")
+    assert info["tiles_interior"] > 0
+
+
+def test_jz_written_where_no_plasma(gpu_device):
+    """Jz written into plasma-free windows (nonzero and -0.0 values) puts those
+    tiles back on the Jz path."""
+    rng = np.random.default_rng(77)
+    p = random_problem(rng, nx=520, ny=300, npml=8, nsteps=24, table=False)
+    jz = np.zeros((p.ny, p.nx), np.float32)
+    jz[100:140, 60:90] = rng.standard_normal((40, 30)).astype(np.float32) * 1e-3
+    jz[200:210, 120:125] = -0.0
+
+    def prepare(ref, plan):
+        before = plan.info("tiles_nojz")
+        ref.write_field(F.JZ, jz)
+        plan.write_field(F.JZ, jz)
+        assert plan.info("tiles_nojz") < before
+
+    _run_pair(p, gpu_device, {"tblock": 8}, "jz write", prepare)
+
+
+def test_nonuniform_den_inside(gpu_device):
+    """den values that differ inside the grid (not only in the CPML strips)
+    send the tiles that load them to the general kernel."""
+    rng = np.random.default_rng(78)
+    p = random_problem(rng, nx=480, ny=400, npml=6, nsteps=20, table=True)
+    p.den_ex = p.den_ex.copy()
+    p.den_hy = p.den_hy.copy()
+    p.den_ex[230] *= np.float32(1.25)
+    p.den_hy[190:193] *= np.float32(0.8)
+    info = _run_pair(p, gpu_device, {"tblock": 6}, "den inside")
+    q = random_problem(np.random.default_rng(78), nx=480, ny=400, npml=6, nsteps=20, table=True)
+    with F.Plan(q, gpu_device) as plan:
+        plan.set_option("tblock", 6)
+        assert plan.info("tiles_cpml") < info["tiles_cpml"]
