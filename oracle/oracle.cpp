@@ -18,6 +18,26 @@
 //   comps     F/kernel.cu:719-759   Dev_AESAComponentDetector
 // and the step order of the host loop F/kernel.cu:2065-2157.
 //
+// What pins each function independently of this file (tests/test_oracle_kat.py
+// unless noted), chosen so that a dropped term, a wrong sign or a shifted
+// index fails at least one:
+//   hx_main, hy_main, ez_main  discrete PEC-cavity eigenmode against the Yee
+//                              dispersion closed form (FP64 to 1e-10), a
+//                              perturbed operator failing it, discrete energy
+//                              conservation, reciprocity in an inhomogeneous box
+//   jz                         uniform-plasma modal recurrence (2x2 closed form,
+//                              collisionless and collisional), plasma cutoff
+//   hx_pml .. ez_pml_y         CPML reflection below -40 dB against a large
+//                              domain, with a planted psi mistake failing it;
+//                              coefficient closed forms (tests/test_setup.py)
+//   ez_main PEC id             PEC material forces Ez = 0
+//   source, detectors, comps   marked-grid brute force of every index rule
+//   whole step                 FP32 within tolerance of FP64; threads bitwise
+// No function here is left without such a check. Left unpinned, by nature:
+// the reference's sm_20 FMA contraction, CUDA 7.5 device cos/pow/exp in its
+// disabled chirp path, and its undefined outputs (out-of-bounds trace row,
+// uninitialised buffers); these are defined here as documented above.
+//
 // Departures, all deliberate:
 //  * The reference launches truncated grids (F/kernel.cu:2028-2030) that can
 //    skip psi cells for some sizes; the loops here cover the kernels' full
