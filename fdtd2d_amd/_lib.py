@@ -118,6 +118,18 @@ SIGNATURES = [
     ("fdtd2d_plan_destroy", C.c_int, [C.c_void_p]),
     ("fdtd2d_plan_time_kernels", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_plan_kernel_ms", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    ("fdtd2d_pool_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_pool_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("fdtd2d_pool_get_info", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    ("fdtd2d_pool_add", C.c_int, [C.c_void_p, C.POINTER(ProblemC), C.POINTER(C.c_int)]),
+    ("fdtd2d_pool_step", C.c_int, [C.c_void_p, C.c_int]),
+    ("fdtd2d_pool_sync", C.c_int, [C.c_void_p]),
+    ("fdtd2d_pool_shot_state", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]),
+    ("fdtd2d_pool_read_detectors", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t, _f, C.c_size_t]),
+    ("fdtd2d_pool_read_field", C.c_int, [C.c_void_p, C.c_int, C.c_int, _f, C.c_size_t]),
+    ("fdtd2d_pool_remove", C.c_int, [C.c_void_p, C.c_int]),
+    ("fdtd2d_pool_destroy", C.c_int, [C.c_void_p]),
 ]
 
 

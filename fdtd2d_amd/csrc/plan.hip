@@ -22,41 +22,19 @@
 
 #include "../../include/fdtd2d.h"
 #include "errors.h"
+#include "plan_impl.h"
 #include "plan_internal.h"
 #include "validate.h"
 
 using fdtd2d::set_error;
+using namespace fdtd2d::detail;
 
-namespace {
+namespace fdtd2d {
+namespace detail {
 
-constexpr size_t kAlign = 256;
-// Scratch rows below each field plane: interior tiles direct the stores of
-// lanes/rows they do not own there, so their loop issues the same memory
-// operations every row (nothing reads these rows).
-constexpr int kTrashRows = 64;
-// Discarded stores (halo lanes, rows outside the band) go to this many
-// 64*C-float slots: few enough to stay resident in L2, so they cost no HBM
-// writes, and spread over enough lines to avoid one hot L2 channel.
-constexpr int kTrashSlots = 64;
-// Steps per launch when none is set: on MI355X the interior kernel is issue
-// bound near 8 levels and memory bound below (8192^2: 81 Gcell/s at 1,
-// ~300 at 4, ~520 at 8 with 1 column per lane and 128-row bands).
-constexpr int kDefaultT = 8;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-struct Layout {
-  int nx, ny, pitch, P, xs;
-  size_t off_fields[2][4];
-  size_t off_psi_y[2][2];  // [set][hxy, ezy]
-  size_t off_psi_x[2][2];  // [set][hyx, ezx]
-  size_t off_med, off_lut;
-  size_t off_col[8];  // den_ex, den_hx, hyx_b, hyx_c, ezx_b, ezx_c, hyx_slot, ezx_slot
-  size_t off_row[8];  // den_ey, den_hy, hxy_b, hxy_c, ezy_b, ezy_c, hxy_slot, ezy_slot
-  size_t off_rowinfo, off_evoff, off_events, off_src, off_det, off_comp;
-  size_t n_events;
-  size_t total;
-};
 
 std::vector<fdtd2d::Event> build_events(const fdtd2d_problem *p, std::vector<int32_t> &off) {
   struct E {
@@ -118,60 +96,17 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
   return L;
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace fdtd2d
 
-struct fdtd2d_plan {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipStream_t own_stream = nullptr;
-  char *base = nullptr;
-  Layout L{};
-  fdtd2d::StepArgs a{};
-  int src = 0;
-  int n = 0;
-  int nsteps = 0;
-  int T = 0;  // 0: default
-  int C = 1;
-  int band = 0;  // 0: automatic
-  int D = 2;     // rows of prefetch per wave (interior kernel); the row loop is unrolled by D
-  int band_cpml = 0;               // rows per CPML-class work-group (0: automatic)
-  double cells_class[2] = {0, 0};  // grid cells owned by interior / CPML-class tiles
-  int hc = 0;                      // rows per CPML-class sub-band in use
-  int n_nojz = 0;                  // interior tiles skipping the Jz update
-  int4 *wg_buf = nullptr;          // device copy of the two work-group lists
-  hipStream_t aux = nullptr;       // CPML-class launches run here, beside the interior ones
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int ndet = 0, ncomp = 0, nsrc = 0;
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-  std::vector<hipEvent_t> spare;
-  double sum_ms = 0;
-  int count = 0;
-  int launches_T[fdtd2d::kMaxT + 1] = {0};
-  // Host facts the tiling needs. Prefix counts of the rows / columns whose
-  // den values differ from the uniform interior ones (the interior kernel
-  // takes those as scalars), and per row the column span [lo, hi] holding
-  // plasma (Ne > 0) or a nonzero Jz: windows outside every span skip the Jz
-  // update (there Jz stays +0 and Ez - Jz = Ez).
-  std::vector<int> srow_pref, scol_pref;
-  std::vector<int> ne_lo, ne_hi, jz_lo, jz_hi;
-  // cells holding a source, detector or probe, sorted by (row, column): the
-  // tiles whose window holds one take the CPML-class kernel, so the interior
-  // kernel carries no event code
-  std::vector<int2> ev_cells;
-};
 
-namespace {
+namespace fdtd2d {
+namespace detail {
 
 int hip_fail(hipError_t e, const char *what) {
   return set_error(FDTD2D_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, what)                 \
-  do {                                      \
-    hipError_t e_ = (expr);                 \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
 
 // Band height: enough bands for >= ~8 waves per SIMD on 256 CUs, at least 32
 // rows, at most 256.
@@ -183,27 +118,8 @@ int pick_band(int nx, int ny, int VW, int T) {
   return std::max(H, T);
 }
 
-// Tile lists of one launch shape, one entry {tx, r0, r1, 0} per wave:
-// [0] interior, [1] CPML class. A (tile, band) whose loads touch a CPML strip
-// or the outer ring is cut into sub-bands of hc rows; the sub-bands that still
-// touch go to the CPML list, runs of the others are merged back into one
-// interior entry. Host-only, so the partition can be checked without a GPU.
-struct Tiling {
-  int T = 0, halo = 0, vw = 0, ntx = 0, H = 0, nty = 0, hc = 0;
-  std::vector<int4> lists[2];  // padded to whole work-groups
-  int n_tiles[2] = {0, 0};
-  int n_nojz = 0;  // interior tiles that skip the Jz update
-  double cells[2] = {0, 0};
-};
 
-struct TileMasks {
-  const int *srow_pref = nullptr, *scol_pref = nullptr;  // [ny+1], [nx+1]
-  const int *ne_lo = nullptr, *ne_hi = nullptr, *jz_lo = nullptr, *jz_hi = nullptr;  // [ny]
-  const int2 *ev = nullptr;  // event cells {column, row}, sorted by (row, column)
-  size_t nev = 0;
-};
-
-Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml, const TileMasks *m = nullptr) {
+Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml, const TileMasks *m) {
   Tiling t;
   t.T = T;
   t.halo = ((T + C - 1) / C) * C;
@@ -317,6 +233,13 @@ int set_tiling(fdtd2d_plan *pl) {
     a.n_tiles[c] = t.n_tiles[c];
   }
   pl->n_nojz = t.n_nojz;
+  if (pl->pooled) {  // the pool concatenates the host lists of its shots
+    a.n_wg[0] = static_cast<int>(t.lists[0].size()) / fdtd2d::kWavesPerWG;
+    a.n_wg[1] = static_cast<int>(t.lists[1].size()) / fdtd2d::kWavesPerWG;
+    a.tile_list[0] = a.tile_list[1] = nullptr;
+    pl->tiling = std::move(t);
+    return FDTD2D_OK;
+  }
   const std::vector<int4> *lists = t.lists;
   const size_t n = lists[0].size() + lists[1].size();
   if (pl->wg_buf) {
@@ -370,7 +293,8 @@ hipEvent_t get_event(fdtd2d_plan *pl) {
   return e;
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace fdtd2d
 
 extern "C" {
 
@@ -436,6 +360,17 @@ int fdtd2d_plan_footprint(const fdtd2d_problem *p, uint64_t *bytes) {
 }
 
 int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
+  return create_plan(p, device, false, nullptr, 0, 1, 0, 0, nullptr, out);
+}
+
+}  // extern "C"
+
+size_t fdtd2d::detail::plan_bytes(const fdtd2d_problem *p) {
+  return make_layout(p, static_cast<size_t>(p->nsrc) + p->ndet + p->ncomp).total;
+}
+
+int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled, hipStream_t stream, int T, int C,
+                                int band, int band_cpml, char *base, fdtd2d_plan **out) {
   if (!out) return set_error(FDTD2D_EINVAL, "null out");
   *out = nullptr;
   int rc = fdtd2d::validate_problem(p);
@@ -452,13 +387,24 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   auto *pl = new (std::nothrow) fdtd2d_plan();
   if (!pl) return set_error(FDTD2D_ENOMEM, "out of host memory");
   pl->device = device;
+  pl->pooled = pooled;
+  pl->T = T;
+  pl->C = C;
+  pl->band = band;
+  pl->band_cpml = band_cpml;
   pl->L = make_layout(p, events.size());
   pl->nsteps = p->nsteps;
   pl->ndet = p->ndet;
   pl->ncomp = p->ncomp;
   pl->nsrc = p->nsrc;
   const Layout &L = pl->L;
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(&pl->base), L.total);
+  hipError_t e = hipSuccess;
+  if (base) {
+    pl->base = base;
+    pl->owns_base = false;
+  } else {
+    e = hipMalloc(reinterpret_cast<void **>(&pl->base), L.total);
+  }
   if (e != hipSuccess) {
     delete pl;
     return set_error(FDTD2D_ENOMEM, std::string("hipMalloc of ") + std::to_string(L.total) +
@@ -468,13 +414,20 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
     fdtd2d_plan_destroy(pl);
     return code;
   };
-  if (hipStreamCreateWithFlags(&pl->own_stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&pl->aux, hipStreamNonBlocking) != hipSuccess)
-    return fail(set_error(FDTD2D_EDEVICE, "hipStreamCreate"));
-  if (hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)
-    return fail(set_error(FDTD2D_EDEVICE, "hipEventCreate"));
-  pl->stream = pl->own_stream;
+  if (pooled) {
+    // a pool orders its shots' work itself (events on its own streams); the
+    // plan's copies go to the null stream, which never waits for the pool's
+    (void)stream;
+    pl->stream = nullptr;
+  } else {
+    if (hipStreamCreateWithFlags(&pl->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&pl->aux, hipStreamNonBlocking) != hipSuccess)
+      return fail(set_error(FDTD2D_EDEVICE, "hipStreamCreate"));
+    if (hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)
+      return fail(set_error(FDTD2D_EDEVICE, "hipEventCreate"));
+    pl->stream = pl->own_stream;
+  }
   if (hipMemset(pl->base, 0, L.total) != hipSuccess) return fail(set_error(FDTD2D_EDEVICE, "hipMemset"));
 
   const int nx = L.nx, ny = L.ny, pitch = L.pitch, P = L.P;
@@ -695,6 +648,8 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   return FDTD2D_OK;
 }
 
+extern "C" {
+
 int fdtd2d_plan_set_stream(fdtd2d_plan *pl, void *stream) {
   if (!pl) return set_error(FDTD2D_EINVAL, "null plan");
   pl->stream = stream ? static_cast<hipStream_t>(stream) : pl->own_stream;
@@ -764,6 +719,7 @@ int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value)
 
 int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
   if (!pl) return set_error(FDTD2D_EINVAL, "null plan");
+  if (pl->pooled) return set_error(FDTD2D_ESTATE, "a pool's shot is stepped by the pool");
   if (nsteps < 0 || pl->n + nsteps > pl->nsteps)
     return set_error(FDTD2D_EINVAL, "run beyond the problem's nsteps (" + std::to_string(pl->nsteps) + ")");
   HIP_TRY(hipSetDevice(pl->device), "hipSetDevice");
@@ -976,7 +932,7 @@ int fdtd2d_plan_destroy(fdtd2d_plan *pl) {
   if (pl->ev_fork) hipEventDestroy(pl->ev_fork);
   if (pl->ev_join) hipEventDestroy(pl->ev_join);
   if (pl->wg_buf) hipFree(pl->wg_buf);
-  if (pl->base) hipFree(pl->base);
+  if (pl->base && pl->owns_base) hipFree(pl->base);
   if (pl->aux) hipStreamDestroy(pl->aux);
   if (pl->own_stream) hipStreamDestroy(pl->own_stream);
   delete pl;

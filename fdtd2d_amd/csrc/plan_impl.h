@@ -1,0 +1,140 @@
+// plan_impl.h - internals shared by plan.hip (single plans) and pool.hip
+// (pools of independent shots advanced by the persistent kernels).
+#ifndef FDTD2D_PLAN_IMPL_H
+#define FDTD2D_PLAN_IMPL_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fdtd2d.h"
+#include "plan_internal.h"
+
+namespace fdtd2d {
+namespace detail {
+
+constexpr size_t kAlign = 256;
+// Scratch rows below each field plane: interior tiles direct the stores of
+// lanes/rows they do not own there, so their loop issues the same memory
+// operations every row (nothing reads these rows).
+constexpr int kTrashRows = 64;
+// Discarded stores (halo lanes, rows outside the band) go to this many
+// 64*C-float slots: few enough to stay resident in L2, so they cost no HBM
+// writes, and spread over enough lines to avoid one hot L2 channel.
+constexpr int kTrashSlots = 64;
+// Steps per launch when none is set: on MI355X the interior kernel is issue
+// bound near 8 levels and memory bound below (8192^2: 81 Gcell/s at 1,
+// ~300 at 4, ~520 at 8 with 1 column per lane and 128-row bands).
+constexpr int kDefaultT = 8;
+
+struct Layout {
+  int nx, ny, pitch, P, xs;
+  size_t off_fields[2][4];
+  size_t off_psi_y[2][2];  // [set][hxy, ezy]
+  size_t off_psi_x[2][2];  // [set][hyx, ezx]
+  size_t off_med, off_lut;
+  size_t off_col[8];  // den_ex, den_hx, hyx_b, hyx_c, ezx_b, ezx_c, hyx_slot, ezx_slot
+  size_t off_row[8];  // den_ey, den_hy, hxy_b, hxy_c, ezy_b, ezy_c, hxy_slot, ezy_slot
+  size_t off_rowinfo, off_evoff, off_events, off_src, off_det, off_comp;
+  size_t n_events;
+  size_t total;
+};
+
+// Tile lists of one launch shape, one entry {tx, r0, r1, 0} per wave:
+// [0] interior, [1] CPML class. A (tile, band) whose loads touch a CPML strip
+// or the outer ring is cut into sub-bands of hc rows; the sub-bands that still
+// touch go to the CPML list, runs of the others are merged back into one
+// interior entry. Host-only, so the partition can be checked without a GPU.
+struct Tiling {
+  int T = 0, halo = 0, vw = 0, ntx = 0, H = 0, nty = 0, hc = 0;
+  std::vector<int4> lists[2];  // padded to whole work-groups
+  int n_tiles[2] = {0, 0};
+  int n_nojz = 0;  // interior tiles that skip the Jz update
+  double cells[2] = {0, 0};
+};
+
+struct TileMasks {
+  const int *srow_pref = nullptr, *scol_pref = nullptr;  // [ny+1], [nx+1]
+  const int *ne_lo = nullptr, *ne_hi = nullptr, *jz_lo = nullptr, *jz_hi = nullptr;  // [ny]
+  const int2 *ev = nullptr;  // event cells {column, row}, sorted by (row, column)
+  size_t nev = 0;
+};
+
+Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml, const TileMasks *m = nullptr);
+Layout make_layout(const fdtd2d_problem *p, size_t n_events);
+int hip_fail(hipError_t e, const char *what);
+
+}  // namespace detail
+}  // namespace fdtd2d
+
+struct fdtd2d_plan {
+  bool pooled = false;  // a pool shot: no streams of its own, host tile lists only
+  bool owns_base = true;  // false: base is carved from a pool's arena
+  fdtd2d::detail::Tiling tiling;  // pooled: the tile lists the pool concatenates
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  char *base = nullptr;
+  fdtd2d::detail::Layout L{};
+  fdtd2d::StepArgs a{};
+  int src = 0;
+  int n = 0;
+  int nsteps = 0;
+  int T = 0;  // 0: default
+  int C = 1;
+  int band = 0;  // 0: automatic
+  int D = 2;     // rows of prefetch per wave (interior kernel); the row loop is unrolled by D
+  int band_cpml = 0;               // rows per CPML-class work-group (0: automatic)
+  double cells_class[2] = {0, 0};  // grid cells owned by interior / CPML-class tiles
+  int hc = 0;                      // rows per CPML-class sub-band in use
+  int n_nojz = 0;                  // interior tiles skipping the Jz update
+  int4 *wg_buf = nullptr;          // device copy of the two work-group lists
+  hipStream_t aux = nullptr;       // CPML-class launches run here, beside the interior ones
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int ndet = 0, ncomp = 0, nsrc = 0;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> spare;
+  double sum_ms = 0;
+  int count = 0;
+  int launches_T[fdtd2d::kMaxT + 1] = {0};
+  // Host facts the tiling needs. Prefix counts of the rows / columns whose
+  // den values differ from the uniform interior ones (the interior kernel
+  // takes those as scalars), and per row the column span [lo, hi] holding
+  // plasma (Ne > 0) or a nonzero Jz: windows outside every span skip the Jz
+  // update (there Jz stays +0 and Ez - Jz = Ez).
+  std::vector<int> srow_pref, scol_pref;
+  std::vector<int> ne_lo, ne_hi, jz_lo, jz_hi;
+  // cells holding a source, detector or probe, sorted by (row, column): the
+  // tiles whose window holds one take the CPML-class kernel, so the interior
+  // kernel carries no event code
+  std::vector<int2> ev_cells;
+};
+
+namespace fdtd2d {
+namespace detail {
+
+// Creates a plan. pooled: no streams or events of its own and tile lists
+// kept on the host for a pool, whose launch shape (T, C, band, band_cpml) it
+// takes; work goes to `stream`.
+// base: device memory of at least make_layout(p, events).total bytes to use
+// instead of a hipMalloc of its own (nullptr: allocate).
+int create_plan(const fdtd2d_problem *p, int device, bool pooled, hipStream_t stream, int T, int C, int band,
+                int band_cpml, char *base, fdtd2d_plan **out);
+// Bytes create_plan needs for p (the plan's own allocation).
+size_t plan_bytes(const fdtd2d_problem *p);
+int set_tiling(fdtd2d_plan *pl);
+int current_T(const fdtd2d_plan *pl);
+
+}  // namespace detail
+}  // namespace fdtd2d
+
+#define HIP_TRY(expr, what)                                            \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess) return ::fdtd2d::detail::hip_fail(e_, what); \
+  } while (0)
+
+#endif

@@ -79,6 +79,24 @@ struct StepArgs {
   const int4 *tile_list[2];
   int n_tiles[2];  // entries in use
   int n_wg[2];     // work-groups launched (entries / 4, rounded up)
+  // pool mode: at pool launch g this shot runs steps n_base + (g - g_base)*tp
+  // onwards from buffer set src_base ^ ((g - g_base) & 1)
+  int g_base, n_base, src_base;
+};
+
+// Work queue of the pool kernels, in device memory. items/count: the packed
+// lists per class (work-groups of 4 tile entries); g: pool launches completed
+// before the current chunk; ticket[k][cls][x]: work-groups taken from XCD
+// range x of class cls by launch k of the chunk (reset by the tick kernel).
+// items, count and g are written by kernels on the pool's stream, so every
+// launch sees the lists in stream order, also inside a replayed graph.
+constexpr int kMaxChunk = 16;
+struct PoolQueue {
+  const int4 *items[2];
+  int count[2];
+  int g;
+  int pad;
+  int ticket[kMaxChunk][2][8];
 };
 
 // One launch advances T steps (1..kMaxT).
@@ -89,10 +107,19 @@ constexpr int kWavesPerWG = 4;
 // for T steps on `stream`, D rows of prefetch; src = buffer set read.
 hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream);
 
-// True when any tile of a work-group (column group wgx, rows [jb0, jb1))
-// loads a cell of a CPML strip.
-// (With no CPML the outer PEC ring counts as the strip: the interior kernel
-// applies no index masks, so it must never load a ring cell.)
+// Pool kernels: launch k of a chunk, class cls, T steps, on `grid`
+// persistent work-groups that take work-groups of tile entries (entry.w =
+// flags | shot slot << 8, one slot per work-group) from q's lists until they
+// are exhausted. tp = the pool's steps per full launch. Only C = 1;
+// D = 1 or 2 for the interior class.
+hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D, int medium_kind,
+                       int grid, hipStream_t stream);
+// Resets the tickets of launches [0, nk) and adds dg to q->g (one wave).
+hipError_t launch_pool_tick(PoolQueue *q, int nk, int dg, hipStream_t stream);
+// Points q at new lists (g >= 0 also sets q->g), in stream order.
+hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,
+                           hipStream_t stream);
+
 // True when tile tx (vw stored columns, halo recomputed columns each side)
 // over output rows [jb0, jb1) loads a cell of a CPML strip or of the outer
 // ring (with no CPML the ring counts as a strip of width 1): such a tile

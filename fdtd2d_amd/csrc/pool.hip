@@ -1,0 +1,719 @@
+// pool.hip - many independent shots (grids of any size, own media, sources,
+// detectors and step counts) advanced together: the batched and served forms
+// of the stepper. No reference counterpart: the reference runs one grid per
+// process (F/FDTD_cuda.cu:384-441).
+//
+// Every pool launch advances each unfinished shot by min(T, steps left).
+// The shots' tile entries are concatenated into two packed lists (interior,
+// CPML class), one work-group of 4 entries per shot at a time, and two
+// persistent kernels (one per class, on two streams) drain them through a
+// ticket queue in device memory. Per-shot parameters live in a descriptor
+// table; a shot's step index and buffer set follow from the pool's launch
+// counter g, which a one-wave tick kernel advances on the device. Nothing a
+// launch reads changes its address, so a chunk of launches can be captured
+// once as a hipGraph and replayed while shots come and go.
+//
+// Shots whose steps left are fewer than T run in separate tail launches of
+// their exact step count at the same g. Device memory of shots comes from one
+// arena when the pool has a byte budget (admission control), else from one
+// hipMalloc per shot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd2d.h"
+#include "errors.h"
+#include "plan_impl.h"
+#include "plan_internal.h"
+#include "validate.h"
+
+using fdtd2d::set_error;
+using namespace fdtd2d::detail;
+
+namespace {
+
+constexpr size_t kArenaAlign = 256;
+
+// First-fit allocator over one device allocation.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  std::map<size_t, size_t> used_;  // offset -> length
+  size_t in_use = 0;
+
+  bool init(size_t bytes) {
+    if (hipMalloc(reinterpret_cast<void **>(&base), bytes) != hipSuccess) return false;
+    size = bytes;
+    free_[0] = bytes;
+    return true;
+  }
+  char *take(size_t bytes) {
+    bytes = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > bytes) free_[off + bytes] = len - bytes;
+      used_[off] = bytes;
+      in_use += bytes;
+      return base + off;
+    }
+    return nullptr;
+  }
+  void give(char *p) {
+    auto it = used_.find(static_cast<size_t>(p - base));
+    if (it == used_.end()) return;
+    size_t off = it->first, len = it->second;
+    used_.erase(it);
+    in_use -= len;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + len == nx->first) {
+      len += nx->second;
+      free_.erase(nx);
+    }
+    auto pv = free_.lower_bound(off);
+    if (pv != free_.begin()) {
+      --pv;
+      if (pv->first + pv->second == off) {
+        off = pv->first;
+        len += pv->second;
+        free_.erase(pv);
+      }
+    }
+    free_[off] = len;
+  }
+  size_t largest_free() const {
+    size_t m = 0;
+    for (auto &f : free_) m = std::max(m, f.second);
+    return m;
+  }
+};
+
+struct Shot {
+  fdtd2d_plan *pl = nullptr;
+  bool live = false;      // slot in use
+  bool finished = false;  // all steps issued
+  int last_g = -1;        // pool launch in which its last steps were issued
+  size_t bytes = 0;
+  int medium_kind = 0;
+};
+
+template <class T>
+int grow(T *&buf, size_t &cap, size_t need, bool &moved) {
+  if (need <= cap) return FDTD2D_OK;
+  size_t nc = std::max<size_t>(need, cap * 2);
+  nc = std::max<size_t>(nc, 1024);
+  T *nb = nullptr;
+  if (hipMalloc(reinterpret_cast<void **>(&nb), nc * sizeof(T)) != hipSuccess)
+    return set_error(FDTD2D_ENOMEM, "hipMalloc pool lists");
+  if (buf) hipFree(buf);
+  buf = nb;
+  cap = nc;
+  moved = true;
+  return FDTD2D_OK;
+}
+
+}  // namespace
+
+struct fdtd2d_pool {
+  int device = 0;
+  int ncu = 256;
+  hipStream_t stream = nullptr, aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // launch shape
+  int T = 8, band = 0, band_cpml = 0, D = 2;
+  int wg_per_cu = 4, wg_per_cu_cpml = 1;
+  int chunk = 8;  // launches per captured graph; 0 = no graphs
+  int medium_kind = -1;
+  // memory
+  size_t budget = 0;
+  Arena arena;
+  // shots
+  std::vector<Shot> shots;
+  std::vector<int> free_slots;
+  // device state
+  fdtd2d::StepArgs *d_descs = nullptr;
+  size_t cap_slots = 0;
+  // main lists, double-buffered: a rebuild writes the buffer the launches in
+  // flight do not read, once the last launch that read it has finished
+  struct ListBuf {
+    int4 *d[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    int last_g = -1;  // last pool launch that read it
+  } bufs[2];
+  int cur = 0;
+  int4 *d_tail[2] = {nullptr, nullptr};
+  size_t cap_tail[2] = {0, 0};
+  fdtd2d::PoolQueue *d_q = nullptr, *d_qt = nullptr;
+  // host bookkeeping
+  int g = 0;                 // pool launches issued
+  std::vector<int> main_set;  // slots in the current main lists
+  bool lists_valid = false;
+  // graph of `chunk` launches
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  // completion events: one recorded after every launch
+  std::vector<std::pair<int, hipEvent_t>> done_events;  // (g after launch, event)
+  std::vector<hipEvent_t> spare_events;
+  int g_done_known = 0;  // launches known complete on the device
+  // stats
+  double cell_updates = 0;
+  int graph_replays = 0, eager_launches = 0, tail_launches = 0;
+};
+
+namespace {
+
+void retire(fdtd2d_pool *pp, int wait_g);
+
+int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
+  std::vector<int4> lists[2];
+  for (int s : slots) {
+    const Tiling &t = pp->shots[s].pl->tiling;
+    for (int c = 0; c < 2; ++c)
+      for (const int4 &e : t.lists[c]) lists[c].push_back(make_int4(e.x, e.y, e.z, (e.w & 0xff) | (s << 8)));
+  }
+  const int nb = 1 - pp->cur;
+  fdtd2d_pool::ListBuf &b = pp->bufs[nb];
+  retire(pp, b.last_g + 1);  // launches that read this buffer have finished
+  bool moved = false;
+  for (int c = 0; c < 2; ++c) {
+    int rc = grow(b.d[c], b.cap[c], std::max<size_t>(lists[c].size(), 4), moved);
+    if (rc) return rc;
+    if (!lists[c].empty())
+      HIP_TRY(hipMemcpy(b.d[c], lists[c].data(), lists[c].size() * sizeof(int4), hipMemcpyHostToDevice),
+              "hipMemcpy pool lists");
+  }
+  hipError_t e = fdtd2d::launch_pool_set(pp->d_q, b.d[0], b.d[1], static_cast<int>(lists[0].size() / 4),
+                                         static_cast<int>(lists[1].size() / 4), -1, pp->stream);
+  if (e != hipSuccess) return hip_fail(e, "pool set launch");
+  pp->cur = nb;
+  pp->main_set = slots;
+  pp->lists_valid = true;
+  return FDTD2D_OK;
+}
+
+int upload_desc(fdtd2d_pool *pp, int slot) {
+  fdtd2d_plan *pl = pp->shots[slot].pl;
+  fdtd2d::StepArgs a = pl->a;
+  a.g_base = pp->g;
+  a.n_base = pl->n;
+  a.src_base = pl->src;
+  HIP_TRY(hipMemcpy(pp->d_descs + slot, &a, sizeof(a), hipMemcpyHostToDevice), "hipMemcpy descriptor");
+  return FDTD2D_OK;
+}
+
+hipEvent_t take_event(fdtd2d_pool *pp) {
+  if (!pp->spare_events.empty()) {
+    hipEvent_t e = pp->spare_events.back();
+    pp->spare_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Retires completion events in launch order: waits for those of launches up
+// to wait_g (launch counts), takes the others only if they have passed.
+void retire(fdtd2d_pool *pp, int wait_g) {
+  size_t k = 0;
+  for (; k < pp->done_events.size(); ++k) {
+    auto &de = pp->done_events[k];
+    if (de.first <= wait_g) hipEventSynchronize(de.second);
+    else if (hipEventQuery(de.second) != hipSuccess) break;
+    pp->g_done_known = std::max(pp->g_done_known, de.first);
+    pp->spare_events.push_back(de.second);
+  }
+  pp->done_events.erase(pp->done_events.begin(), pp->done_events.begin() + k);
+}
+
+int record_done(fdtd2d_pool *pp) {
+  hipEvent_t e = take_event(pp);
+  if (!e) return set_error(FDTD2D_EDEVICE, "hipEventCreate");
+  HIP_TRY(hipEventRecord(e, pp->stream), "hipEventRecord");
+  pp->done_events.push_back({pp->g, e});
+  if (pp->done_events.size() > 256) retire(pp, -1);
+  return FDTD2D_OK;
+}
+
+// One pool launch of the main lists at chunk position k (both classes).
+int launch_main(fdtd2d_pool *pp, int k) {
+  const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
+  HIP_TRY(hipEventRecord(pp->ev_fork, pp->stream), "hipEventRecord fork");
+  HIP_TRY(hipStreamWaitEvent(pp->aux, pp->ev_fork, 0), "hipStreamWaitEvent fork");
+  hipError_t e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 1, pp->T, 1, pp->medium_kind, gc, pp->aux);
+  if (e != hipSuccess) return hip_fail(e, "pool CPML kernel launch");
+  HIP_TRY(hipEventRecord(pp->ev_join, pp->aux), "hipEventRecord join");
+  e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 0, pp->T, pp->D, pp->medium_kind, gi, pp->stream);
+  if (e != hipSuccess) return hip_fail(e, "pool kernel launch");
+  HIP_TRY(hipStreamWaitEvent(pp->stream, pp->ev_join, 0), "hipStreamWaitEvent join");
+  return FDTD2D_OK;
+}
+
+int capture_graph(fdtd2d_pool *pp) {
+  if (pp->gexec) {
+    hipGraphExecDestroy(pp->gexec);
+    pp->gexec = nullptr;
+  }
+  HIP_TRY(hipStreamBeginCapture(pp->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  int rc = FDTD2D_OK;
+  for (int k = 0; k < pp->chunk && !rc; ++k) rc = launch_main(pp, k);
+  if (!rc) {
+    hipError_t e = fdtd2d::launch_pool_tick(pp->d_q, pp->chunk, pp->chunk, pp->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "pool tick launch");
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(pp->stream, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&pp->gexec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  pp->graph_valid = true;
+  return FDTD2D_OK;
+}
+
+// Tail launches at the current g: shots with 0 < steps left < T, grouped by
+// steps left.
+int run_tails(fdtd2d_pool *pp, const std::vector<int> &tails) {
+  std::map<int, std::vector<int>> by_t;
+  for (int s : tails) by_t[pp->shots[s].pl->nsteps - pp->shots[s].pl->n].push_back(s);
+  for (auto &kv : by_t) {
+    const int t = kv.first;
+    std::vector<int4> lists[2];
+    for (int s : kv.second) {
+      const Tiling &tl = pp->shots[s].pl->tiling;
+      for (int c = 0; c < 2; ++c)
+        for (const int4 &e : tl.lists[c]) lists[c].push_back(make_int4(e.x, e.y, e.z, (e.w & 0xff) | (s << 8)));
+    }
+    // the tail buffers are reused: wait until the previous tail launch is done
+    HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+    bool moved = false;
+    for (int c = 0; c < 2; ++c) {
+      int rc = grow(pp->d_tail[c], pp->cap_tail[c], std::max<size_t>(lists[c].size(), 4), moved);
+      if (rc) return rc;
+      if (!lists[c].empty())
+        HIP_TRY(hipMemcpy(pp->d_tail[c], lists[c].data(), lists[c].size() * sizeof(int4), hipMemcpyHostToDevice),
+                "hipMemcpy tail lists");
+    }
+    const int n0 = static_cast<int>(lists[0].size() / 4), n1 = static_cast<int>(lists[1].size() / 4);
+    const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
+    hipError_t e = fdtd2d::launch_pool_set(pp->d_qt, pp->d_tail[0], pp->d_tail[1], n0, n1, pp->g, pp->stream);
+    if (e == hipSuccess && n1 > 0)
+      e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 1, t, 1, pp->medium_kind, std::min(gc, n1), pp->stream);
+    if (e == hipSuccess && n0 > 0)
+      e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 0, t, pp->D, pp->medium_kind, std::min(gi, n0),
+                              pp->stream);
+    if (e == hipSuccess) e = fdtd2d::launch_pool_tick(pp->d_qt, 1, 0, pp->stream);
+    if (e != hipSuccess) return hip_fail(e, "pool tail launch");
+    pp->tail_launches++;
+    for (int s : kv.second) {
+      fdtd2d_plan *pl = pp->shots[s].pl;
+      pp->cell_updates += static_cast<double>(pl->L.nx) * pl->L.ny * t;
+      pl->n += t;
+      pl->src ^= 1;
+      pp->shots[s].finished = true;
+      pp->shots[s].last_g = pp->g;
+    }
+  }
+  return FDTD2D_OK;
+}
+
+int reshape(fdtd2d_pool *pp) {
+  for (size_t s = 0; s < pp->shots.size(); ++s) {
+    Shot &sh = pp->shots[s];
+    if (!sh.live) continue;
+    sh.pl->T = pp->T;
+    sh.pl->band = pp->band;
+    sh.pl->band_cpml = pp->band_cpml;
+    int rc = set_tiling(sh.pl);
+    if (rc) return rc;
+    if ((rc = upload_desc(pp, static_cast<int>(s)))) return rc;
+  }
+  pp->lists_valid = false;
+  pp->graph_valid = false;
+  return FDTD2D_OK;
+}
+
+int step_impl(fdtd2d_pool *pp, int nlaunch) {
+  HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+  while (nlaunch > 0) {
+    std::vector<int> main_set, tails;
+    int min_left = 1 << 30;
+    for (size_t s = 0; s < pp->shots.size(); ++s) {
+      const Shot &sh = pp->shots[s];
+      if (!sh.live || sh.finished) continue;
+      const int left = sh.pl->nsteps - sh.pl->n;
+      if (left >= pp->T) {
+        main_set.push_back(static_cast<int>(s));
+        min_left = std::min(min_left, left);
+      } else {
+        tails.push_back(static_cast<int>(s));
+      }
+    }
+    if (main_set.empty() && tails.empty()) break;
+    if (!pp->lists_valid || main_set != pp->main_set) {
+      int rc = rebuild_main(pp, main_set);
+      if (rc) return rc;
+    }
+    const bool use_graph = pp->chunk > 0 && nlaunch >= pp->chunk && tails.empty() && !main_set.empty() &&
+                           min_left >= pp->chunk * pp->T;
+    if (use_graph) {
+      if (!pp->graph_valid) {
+        int rc = capture_graph(pp);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(pp->gexec, pp->stream), "hipGraphLaunch");
+      pp->graph_replays++;
+      pp->bufs[pp->cur].last_g = pp->g + pp->chunk - 1;
+      for (int s : main_set) {
+        fdtd2d_plan *pl = pp->shots[s].pl;
+        pp->cell_updates += static_cast<double>(pl->L.nx) * pl->L.ny * pp->T * pp->chunk;
+        pl->n += pp->T * pp->chunk;
+        pl->src ^= pp->chunk & 1;
+        if (pl->n >= pl->nsteps) {
+          pp->shots[s].finished = true;
+          pp->shots[s].last_g = pp->g + pp->chunk - 1;
+        }
+      }
+      pp->g += pp->chunk;
+      nlaunch -= pp->chunk;
+    } else {
+      if (!main_set.empty()) {
+        int rc = launch_main(pp, 0);
+        if (rc) return rc;
+        pp->eager_launches++;
+        pp->bufs[pp->cur].last_g = pp->g;
+      }
+      if (!tails.empty()) {
+        int rc = run_tails(pp, tails);
+        if (rc) return rc;
+      }
+      hipError_t e = fdtd2d::launch_pool_tick(pp->d_q, 1, 1, pp->stream);
+      if (e != hipSuccess) return hip_fail(e, "pool tick launch");
+      for (int s : main_set) {
+        fdtd2d_plan *pl = pp->shots[s].pl;
+        pp->cell_updates += static_cast<double>(pl->L.nx) * pl->L.ny * pp->T;
+        pl->n += pp->T;
+        pl->src ^= 1;
+        if (pl->n >= pl->nsteps) {
+          pp->shots[s].finished = true;
+          pp->shots[s].last_g = pp->g;
+        }
+      }
+      pp->g += 1;
+      nlaunch -= 1;
+    }
+    int rc = record_done(pp);
+    if (rc) return rc;
+  }
+  return FDTD2D_OK;
+}
+
+Shot *get_shot(fdtd2d_pool *pp, int id) {
+  if (!pp || id < 0 || id >= static_cast<int>(pp->shots.size()) || !pp->shots[id].live) return nullptr;
+  return &pp->shots[id];
+}
+
+// Waits until the launch that issued the shot's last steps (or, for a shot
+// still running, everything issued) has finished on the device.
+int wait_shot(fdtd2d_pool *pp, const Shot &sh) {
+  if (sh.finished) {
+    retire(pp, sh.last_g + 1);
+    if (pp->g_done_known > sh.last_g) return FDTD2D_OK;
+  }
+  HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+  retire(pp, -1);
+  return FDTD2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdtd2d_pool_create(int device, fdtd2d_pool **out) {
+  if (!out) return set_error(FDTD2D_EINVAL, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+  if (device < 0 || device >= ndev) return set_error(FDTD2D_EINVAL, "no such device");
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  auto *pp = new (std::nothrow) fdtd2d_pool();
+  if (!pp) return set_error(FDTD2D_ENOMEM, "out of host memory");
+  pp->device = device;
+  hipDeviceGetAttribute(&pp->ncu, hipDeviceAttributeMultiprocessorCount, device);
+  if (pp->ncu <= 0) pp->ncu = 256;
+  auto fail = [&](int rc) {
+    fdtd2d_pool_destroy(pp);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&pp->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&pp->aux, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(FDTD2D_EDEVICE, "hipStreamCreate"));
+  if (hipEventCreateWithFlags(&pp->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&pp->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(set_error(FDTD2D_EDEVICE, "hipEventCreate"));
+  if (hipMalloc(reinterpret_cast<void **>(&pp->d_q), sizeof(fdtd2d::PoolQueue)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&pp->d_qt), sizeof(fdtd2d::PoolQueue)) != hipSuccess)
+    return fail(set_error(FDTD2D_ENOMEM, "hipMalloc pool queue"));
+  if (hipMemset(pp->d_q, 0, sizeof(fdtd2d::PoolQueue)) != hipSuccess ||
+      hipMemset(pp->d_qt, 0, sizeof(fdtd2d::PoolQueue)) != hipSuccess)
+    return fail(set_error(FDTD2D_EDEVICE, "hipMemset"));
+  *out = pp;
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
+  if (!pp || !name) return set_error(FDTD2D_EINVAL, "null argument");
+  const std::string n = name;
+  const long long v = static_cast<long long>(value);
+  bool shape = false;
+  if (n == "budget_bytes") {
+    if (!pp->shots.empty() || pp->arena.base) return set_error(FDTD2D_ESTATE, "budget must be set before any shot");
+    if (value < 0) return set_error(FDTD2D_EINVAL, "negative budget");
+    pp->budget = static_cast<size_t>(value);
+    if (pp->budget > 0) {
+      HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+      if (!pp->arena.init(pp->budget)) return set_error(FDTD2D_ENOMEM, "hipMalloc of the pool arena");
+    }
+    return FDTD2D_OK;
+  }
+  if (n == "tblock") {
+    if (v < 1 || v > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "tblock must be 1..8");
+    pp->T = static_cast<int>(v);
+    shape = true;
+  } else if (n == "band") {
+    if (v < 0) return set_error(FDTD2D_EINVAL, "band must be >= 0");
+    pp->band = static_cast<int>(v);
+    shape = true;
+  } else if (n == "band_cpml") {
+    if (v < 0) return set_error(FDTD2D_EINVAL, "band_cpml must be >= 0");
+    pp->band_cpml = static_cast<int>(v);
+    shape = true;
+  } else if (n == "prefetch") {
+    if (v != 1 && v != 2) return set_error(FDTD2D_EINVAL, "prefetch must be 1 or 2");
+    pp->D = static_cast<int>(v);
+    pp->graph_valid = false;
+  } else if (n == "wg_per_cu") {
+    if (v < 1 || v > 8) return set_error(FDTD2D_EINVAL, "wg_per_cu must be 1..8");
+    pp->wg_per_cu = static_cast<int>(v);
+    pp->graph_valid = false;
+  } else if (n == "wg_per_cu_cpml") {
+    if (v < 1 || v > 8) return set_error(FDTD2D_EINVAL, "wg_per_cu_cpml must be 1..8");
+    pp->wg_per_cu_cpml = static_cast<int>(v);
+    pp->graph_valid = false;
+  } else if (n == "graph_chunk") {
+    if (v < 0 || v > fdtd2d::kMaxChunk) return set_error(FDTD2D_EINVAL, "graph_chunk must be 0..16");
+    pp->chunk = static_cast<int>(v);
+    pp->graph_valid = false;
+  } else {
+    return set_error(FDTD2D_EINVAL, "unknown pool option " + n);
+  }
+  if (shape) {
+    HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+    return reshape(pp);
+  }
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value) {
+  if (!pp || !name || !value) return set_error(FDTD2D_EINVAL, "null argument");
+  const std::string n = name;
+  int live = 0, running = 0;
+  for (const Shot &s : pp->shots) {
+    live += s.live;
+    running += s.live && !s.finished;
+  }
+  if (n == "tblock") *value = pp->T;
+  else if (n == "band") *value = pp->band;
+  else if (n == "band_cpml") *value = pp->band_cpml;
+  else if (n == "prefetch") *value = pp->D;
+  else if (n == "wg_per_cu") *value = pp->wg_per_cu;
+  else if (n == "wg_per_cu_cpml") *value = pp->wg_per_cu_cpml;
+  else if (n == "graph_chunk") *value = pp->chunk;
+  else if (n == "budget_bytes") *value = static_cast<double>(pp->budget);
+  else if (n == "bytes_in_use") *value = static_cast<double>(pp->arena.in_use);
+  else if (n == "largest_free") *value = static_cast<double>(pp->budget ? pp->arena.largest_free() : 0);
+  else if (n == "shots") *value = live;
+  else if (n == "running") *value = running;
+  else if (n == "launches") *value = pp->g;
+  else if (n == "cell_updates") *value = pp->cell_updates;
+  else if (n == "graph_replays") *value = pp->graph_replays;
+  else if (n == "eager_launches") *value = pp->eager_launches;
+  else if (n == "tail_launches") *value = pp->tail_launches;
+  else if (n == "cus") *value = pp->ncu;
+  else return set_error(FDTD2D_EINVAL, "unknown pool info " + n);
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
+  if (!pp || !p || !shot) return set_error(FDTD2D_EINVAL, "null argument");
+  *shot = -1;
+  int rc = fdtd2d::validate_problem(p);
+  if (rc) return rc;
+  if (pp->medium_kind >= 0 && p->medium_kind != pp->medium_kind)
+    return set_error(FDTD2D_EUNSUPPORTED, "a pool holds shots of one medium kind");
+  HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+  const size_t bytes = plan_bytes(p);
+  char *base = nullptr;
+  if (pp->budget > 0) {
+    base = pp->arena.take(bytes);
+    if (!base) return set_error(FDTD2D_ENOMEM, "over the pool's memory budget");
+  }
+  fdtd2d_plan *pl = nullptr;
+  rc = create_plan(p, pp->device, true, pp->stream, pp->T, 1, pp->band, pp->band_cpml, base, &pl);
+  if (rc) {
+    if (base) pp->arena.give(base);
+    return rc;
+  }
+  int slot;
+  if (!pp->free_slots.empty()) {
+    slot = pp->free_slots.back();
+    pp->free_slots.pop_back();
+  } else {
+    slot = static_cast<int>(pp->shots.size());
+    if (slot >= (1 << 23)) {
+      fdtd2d_plan_destroy(pl);
+      if (base) pp->arena.give(base);
+      return set_error(FDTD2D_ENOMEM, "too many shots");
+    }
+    pp->shots.emplace_back();
+  }
+  if (static_cast<size_t>(slot) >= pp->cap_slots) {
+    // grow the descriptor table (keeps the existing descriptors)
+    HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+    const size_t nc = std::max<size_t>(64, pp->cap_slots * 2);
+    fdtd2d::StepArgs *nd = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&nd), nc * sizeof(fdtd2d::StepArgs)) != hipSuccess)
+      return set_error(FDTD2D_ENOMEM, "hipMalloc descriptors");
+    if (pp->d_descs) {
+      HIP_TRY(hipMemcpy(nd, pp->d_descs, pp->cap_slots * sizeof(fdtd2d::StepArgs), hipMemcpyDeviceToDevice),
+              "hipMemcpy descriptors");
+      hipFree(pp->d_descs);
+    }
+    pp->d_descs = nd;
+    pp->cap_slots = nc;
+    pp->graph_valid = false;
+  }
+  Shot &sh = pp->shots[slot];
+  sh = Shot();
+  sh.pl = pl;
+  sh.live = true;
+  sh.bytes = bytes;
+  sh.medium_kind = p->medium_kind;
+  sh.finished = p->nsteps == 0;
+  sh.last_g = pp->g - 1;
+  pp->medium_kind = p->medium_kind;
+  if ((rc = upload_desc(pp, slot))) return rc;
+  *shot = slot;
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_step(fdtd2d_pool *pp, int nlaunch) {
+  if (!pp) return set_error(FDTD2D_EINVAL, "null pool");
+  if (nlaunch < 0) return set_error(FDTD2D_EINVAL, "negative launch count");
+  return step_impl(pp, nlaunch);
+}
+
+int fdtd2d_pool_sync(fdtd2d_pool *pp) {
+  if (!pp) return set_error(FDTD2D_EINVAL, "null pool");
+  HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+  retire(pp, pp->g);
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_shot_state(fdtd2d_pool *pp, int id, int *steps_done, int *steps_total, int *complete) {
+  Shot *sh = get_shot(pp, id);
+  if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");
+  if (steps_done) *steps_done = sh->pl->n;
+  if (steps_total) *steps_total = sh->pl->nsteps;
+  if (complete) {
+    retire(pp, -1);
+    *complete = sh->finished && pp->g_done_known > sh->last_g;
+  }
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_read_detectors(fdtd2d_pool *pp, int id, float *det, size_t det_count, float *comp,
+                               size_t comp_count) {
+  Shot *sh = get_shot(pp, id);
+  if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");
+  int rc = wait_shot(pp, *sh);
+  if (rc) return rc;
+  return fdtd2d_read_detectors(sh->pl, det, det_count, comp, comp_count);
+}
+
+int fdtd2d_pool_read_field(fdtd2d_pool *pp, int id, int which, float *out, size_t count) {
+  Shot *sh = get_shot(pp, id);
+  if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");
+  int rc = wait_shot(pp, *sh);
+  if (rc) return rc;
+  return fdtd2d_read_field(sh->pl, which, out, count);
+}
+
+int fdtd2d_pool_remove(fdtd2d_pool *pp, int id) {
+  Shot *sh = get_shot(pp, id);
+  if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");
+  HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+  // A running shot may be named by launches in flight: wait for them.
+  int rc = wait_shot(pp, *sh);
+  if (rc) return rc;
+  if (!sh->finished) {
+    HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+  }
+  char *base = sh->pl->owns_base ? nullptr : sh->pl->base;
+  fdtd2d_plan *pl = sh->pl;
+  sh->pl = nullptr;
+  sh->live = false;
+  // the plan's own teardown syncs the pool stream only when it owns memory
+  pl->stream = nullptr;
+  fdtd2d_plan_destroy(pl);
+  if (base) pp->arena.give(base);
+  pp->free_slots.push_back(id);
+  auto it = std::find(pp->main_set.begin(), pp->main_set.end(), id);
+  if (it != pp->main_set.end()) pp->lists_valid = false;
+  return FDTD2D_OK;
+}
+
+int fdtd2d_pool_destroy(fdtd2d_pool *pp) {
+  if (!pp) return FDTD2D_OK;
+  hipSetDevice(pp->device);
+  if (pp->stream) hipStreamSynchronize(pp->stream);
+  if (pp->aux) hipStreamSynchronize(pp->aux);
+  for (Shot &s : pp->shots)
+    if (s.live && s.pl) {
+      s.pl->stream = nullptr;
+      fdtd2d_plan_destroy(s.pl);
+    }
+  for (auto &de : pp->done_events) hipEventDestroy(de.second);
+  for (auto e : pp->spare_events) hipEventDestroy(e);
+  if (pp->gexec) hipGraphExecDestroy(pp->gexec);
+  for (int c = 0; c < 2; ++c) {
+    for (int b = 0; b < 2; ++b)
+      if (pp->bufs[b].d[c]) hipFree(pp->bufs[b].d[c]);
+    if (pp->d_tail[c]) hipFree(pp->d_tail[c]);
+  }
+  if (pp->d_descs) hipFree(pp->d_descs);
+  if (pp->d_q) hipFree(pp->d_q);
+  if (pp->d_qt) hipFree(pp->d_qt);
+  if (pp->arena.base) hipFree(pp->arena.base);
+  if (pp->ev_fork) hipEventDestroy(pp->ev_fork);
+  if (pp->ev_join) hipEventDestroy(pp->ev_join);
+  if (pp->aux) hipStreamDestroy(pp->aux);
+  if (pp->stream) hipStreamDestroy(pp->stream);
+  delete pp;
+  return FDTD2D_OK;
+}
+
+}  // extern "C"

@@ -54,6 +54,25 @@ __device__ __forceinline__ int cld(const int *p, long long i) { return ((cint_pt
 __device__ __forceinline__ float cld(const float *p, long long i) { return ((cfloat_ptr)p)[i]; }
 __device__ __forceinline__ int cld_i(const void *p, long long i) { return ((cint_ptr)p)[i]; }
 
+// Device pointers held in StepArgs are plain pointers; dereferences go through
+// the global address space explicitly, so that pointers read from a pool's
+// descriptor table (not kernel arguments) still give global, not flat, memory
+// instructions.
+typedef float __attribute__((address_space(1))) gfloat;
+typedef int __attribute__((address_space(1))) gint;
+typedef char __attribute__((address_space(1))) gchar;
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef v2f __attribute__((address_space(1))) gv2f;
+typedef v4f __attribute__((address_space(1))) gv4f;
+__device__ __forceinline__ gfloat *gp(float *p) { return (gfloat *)p; }
+__device__ __forceinline__ const gfloat *gp(const float *p) { return (const gfloat *)p; }
+__device__ __forceinline__ const gint *gp(const int *p) { return (const gint *)p; }
+__device__ __forceinline__ float4 gload4(const float4 *p, long long i) {
+  const v4f t = ((const gv4f *)p)[i];
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+
 // x-neighbours across lanes with wavefront-wide DPP shifts (GFX9 wave_shl:1
 // and wave_shr:1): lane l receives lane l+1 / l-1. The end lanes receive 0;
 // they lie in the tile's halo, whose columns the levels give up one by one.
@@ -251,12 +270,12 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
         o.ez[c] = o.ez[c] - cld(a.src_table, static_cast<long long>(step) * a.nsrc + ev.idx);
       } else if (own_row && ev.i >= own0 && ev.i < own1) {
         if (ev.kind == EV_DET) {
-          a.det[static_cast<long long>(step) * a.ndet + ev.idx] = o.ez[c];
+          gp(a.det)[static_cast<long long>(step) * a.ndet + ev.idx] = o.ez[c];
         } else {
           const long long q = static_cast<long long>(step) * a.ncomp + ev.idx;
-          a.comp[q] = o.ez[c];
-          a.comp[a.comp_plane + q] = o.hx[c];
-          a.comp[2 * a.comp_plane + q] = o.hy[c];
+          gp(a.comp)[q] = o.ez[c];
+          gp(a.comp)[a.comp_plane + q] = o.hx[c];
+          gp(a.comp)[2 * a.comp_plane + q] = o.hy[c];
         }
       }
     }
@@ -268,20 +287,20 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
 // byte offset shared by all streams): one VGPR offset per row, 2 SGPRs per
 // stream, and the loads take the global_load v, voff, s[base] form.
 struct Streams {
-  const char *ez, *hx, *hy, *jz, *med;
+  const gchar *ez, *hx, *hy, *jz, *med;
 };
 
 template <int C>
-__device__ __forceinline__ void oload(const char *base, unsigned off, float (&v)[C]) {
-  const char *q = base + off;
+__device__ __forceinline__ void oload(const gchar *base, unsigned off, float (&v)[C]) {
+  const gchar *q = base + off;
   if constexpr (C == 1) {
-    v[0] = *reinterpret_cast<const float *>(q);
+    v[0] = *(const gfloat *)q;
   } else if constexpr (C == 2) {
-    const float2 t = *reinterpret_cast<const float2 *>(q);
+    const v2f t = *(const gv2f *)q;
     v[0] = t.x;
     v[1] = t.y;
   } else {
-    const float4 t = *reinterpret_cast<const float4 *>(q);
+    const v4f t = *(const gv4f *)q;
     v[0] = t.x;
     v[1] = t.y;
     v[2] = t.z;
@@ -290,30 +309,30 @@ __device__ __forceinline__ void oload(const char *base, unsigned off, float (&v)
 }
 
 template <int C>
-__device__ __forceinline__ void ostore(const char *base, unsigned off, const float (&v)[C]) {
-  char *q = const_cast<char *>(base) + off;
+__device__ __forceinline__ void ostore(const gchar *base, unsigned off, const float (&v)[C]) {
+  gchar *q = const_cast<gchar *>(base) + off;
   if constexpr (C == 1) {
-    *reinterpret_cast<float *>(q) = v[0];
+    *(gfloat *)q = v[0];
   } else if constexpr (C == 2) {
-    *reinterpret_cast<float2 *>(q) = make_float2(v[0], v[1]);
+    *(gv2f *)q = v2f{v[0], v[1]};
   } else {
-    *reinterpret_cast<float4 *>(q) = make_float4(v[0], v[1], v[2], v[3]);
+    *(gv4f *)q = v4f{v[0], v[1], v[2], v[3]};
   }
 }
 
 template <int C>
-__device__ __forceinline__ void load_row(const float *__restrict__ base, int k, int c0, int pitch,
+__device__ __forceinline__ void load_row(const float *base_, int k, int c0, int pitch,
                                          bool ok, float (&v)[C]) {
   if (ok) {
-    const float *q = base + static_cast<long long>(k) * pitch + c0;
+    const gfloat *q = gp(base_) + static_cast<long long>(k) * pitch + c0;
     if constexpr (C == 1) {
       v[0] = q[0];
     } else if constexpr (C == 2) {
-      const float2 t = *reinterpret_cast<const float2 *>(q);
+      const v2f t = *(const gv2f *)q;
       v[0] = t.x;
       v[1] = t.y;
     } else {
-      const float4 t = *reinterpret_cast<const float4 *>(q);
+      const v4f t = *(const gv4f *)q;
       v[0] = t.x;
       v[1] = t.y;
       v[2] = t.z;
@@ -328,16 +347,16 @@ __device__ __forceinline__ void load_row(const float *__restrict__ base, int k, 
 // Stores the C values of a lane at columns [c0, c0+C) of row r, only those in
 // [own0, own1): one vector store when all are owned.
 template <int C>
-__device__ __forceinline__ void store_row(float *__restrict__ base, long long rowp, int c0, int own0, int own1,
+__device__ __forceinline__ void store_row(float *base_, long long rowp, int c0, int own0, int own1,
                                           const float (&v)[C]) {
-  float *q = base + rowp + c0;
+  gfloat *q = gp(base_) + rowp + c0;
   if (c0 >= own0 && c0 + C <= own1) {
     if constexpr (C == 1) {
       q[0] = v[0];
     } else if constexpr (C == 2) {
-      *reinterpret_cast<float2 *>(q) = make_float2(v[0], v[1]);
+      *(gv2f *)q = v2f{v[0], v[1]};
     } else {
-      *reinterpret_cast<float4 *>(q) = make_float4(v[0], v[1], v[2], v[3]);
+      *(gv4f *)q = v4f{v[0], v[1], v[2], v[3]};
     }
   } else {
 #pragma unroll
@@ -390,8 +409,8 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
 #pragma unroll
       for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
     }
-    const int sh = rv ? a.hxy_slot[k] : -1;
-    const int se = rv ? a.ezy_slot[k] : -1;
+    const int sh = rv ? cld(a.hxy_slot, k) : -1;
+    const int se = rv ? cld(a.ezy_slot, k) : -1;
     if (sh >= 0)
       load_row<C>(a.psi_hxy[src] + static_cast<long long>(sh) * pitch, 0, cl.c0, pitch, lane_in, in.phxy);
     else
@@ -405,8 +424,8 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
     const long long rowx = static_cast<long long>(k) * a.xs;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      in.phyx[c] = (rv && cl.hyx_s[c] >= 0) ? a.psi_hyx[src][rowx + cl.hyx_s[c]] : 0.f;
-      in.pezx[c] = (rv && cl.ezx_s[c] >= 0) ? a.psi_ezx[src][rowx + cl.ezx_s[c]] : 0.f;
+      in.phyx[c] = (rv && cl.hyx_s[c] >= 0) ? gp(a.psi_hyx[src])[rowx + cl.hyx_s[c]] : 0.f;
+      in.pezx[c] = (rv && cl.ezx_s[c] >= 0) ? gp(a.psi_ezx[src])[rowx + cl.ezx_s[c]] : 0.f;
     }
   }
 }
@@ -481,8 +500,8 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
       for (int c = 0; c < C; ++c) {
         const int col = cl.c0 + c;
         if (col < own0 || col >= own1) continue;
-        if (cl.hyx_s[c] >= 0) a.psi_hyx[1 - src][rowx + cl.hyx_s[c]] = o.phyx[c];
-        if (cl.ezx_s[c] >= 0) a.psi_ezx[1 - src][rowx + cl.ezx_s[c]] = o.pezx[c];
+        if (cl.hyx_s[c] >= 0) gp(a.psi_hyx[1 - src])[rowx + cl.hyx_s[c]] = o.phyx[c];
+        if (cl.ezx_s[c] >= 0) gp(a.psi_ezx[1 - src])[rowx + cl.ezx_s[c]] = o.pezx[c];
       }
     }
   }
@@ -510,26 +529,26 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
     const int col = cl.c0 + c;
     const bool in = col >= 0 && col < pitch;
     if constexpr (EDGE) {
-      cl.den_ex[c] = in ? a.den_ex[col] : 0.f;
-      cl.den_hx[c] = in ? a.den_hx[col] : 0.f;
-      cl.hyx_s[c] = in ? a.hyx_slot[col] : -1;
-      cl.ezx_s[c] = in ? a.ezx_slot[col] : -1;
-      cl.hyx_b[c] = in ? a.hyx_b[col] : 0.f;
-      cl.hyx_c[c] = in ? a.hyx_c[col] : 0.f;
-      cl.ezx_b[c] = in ? a.ezx_b[col] : 0.f;
-      cl.ezx_c[c] = in ? a.ezx_c[col] : 0.f;
+      cl.den_ex[c] = in ? gp(a.den_ex)[col] : 0.f;
+      cl.den_hx[c] = in ? gp(a.den_hx)[col] : 0.f;
+      cl.hyx_s[c] = in ? gp(a.hyx_slot)[col] : -1;
+      cl.ezx_s[c] = in ? gp(a.ezx_slot)[col] : -1;
+      cl.hyx_b[c] = in ? gp(a.hyx_b)[col] : 0.f;
+      cl.hyx_c[c] = in ? gp(a.hyx_c)[col] : 0.f;
+      cl.ezx_b[c] = in ? gp(a.ezx_b)[col] : 0.f;
+      cl.ezx_c[c] = in ? gp(a.ezx_c)[col] : 0.f;
     }
   }
   Streams sm, dst;
-  sm.ez = reinterpret_cast<const char *>(a.ez[src]);
-  sm.hx = reinterpret_cast<const char *>(a.hx[src]);
-  sm.hy = reinterpret_cast<const char *>(a.hy[src]);
-  sm.jz = reinterpret_cast<const char *>(a.jz[src]);
-  sm.med = MED == kTable ? reinterpret_cast<const char *>(a.med_id) : reinterpret_cast<const char *>(a.med_ne);
-  dst.ez = reinterpret_cast<const char *>(a.ez[1 - src]);
-  dst.hx = reinterpret_cast<const char *>(a.hx[1 - src]);
-  dst.hy = reinterpret_cast<const char *>(a.hy[1 - src]);
-  dst.jz = reinterpret_cast<const char *>(a.jz[1 - src]);
+  sm.ez = (const gchar *)a.ez[src];
+  sm.hx = (const gchar *)a.hx[src];
+  sm.hy = (const gchar *)a.hy[src];
+  sm.jz = (const gchar *)a.jz[src];
+  sm.med = MED == kTable ? (const gchar *)a.med_id : (const gchar *)a.med_ne;
+  dst.ez = (const gchar *)a.ez[1 - src];
+  dst.hx = (const gchar *)a.hx[1 - src];
+  dst.hy = (const gchar *)a.hy[1 - src];
+  dst.jz = (const gchar *)a.jz[1 - src];
   dst.med = sm.med;
   Level<C, EDGE> st[T + 1];
   RowS<EDGE> rs[T + 1];
@@ -576,7 +595,7 @@ template <int T, int C, int D, int MED, bool EDGE>
 __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a, int src, int n0) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   if constexpr (MED == kTable) {
-    for (int q = threadIdx.x; q < a.nmat; q += blockDim.x) lut_s[q] = a.lut[q];
+    for (int q = threadIdx.x; q < a.nmat; q += blockDim.x) lut_s[q] = gload4(a.lut, q);
     __syncthreads();
   }
   // Launch index -> list position: consecutive positions go to one XCD
@@ -595,6 +614,113 @@ __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a
     run_tile<T, C, D, EDGE, MED, true>(a, lut_s, src, n0, tx, r0, r1);
   else
     run_tile<T, C, D, EDGE, MED, false>(a, lut_s, src, n0, tx, r0, r1);
+}
+
+// Persistent form for pools of independent shots: each work-group takes
+// work-groups of 4 tile entries from the XCD range its index maps to (the
+// lists are ordered shot by shot, so an XCD keeps to a few shots and their
+// halo rows stay in its L2), then steals from the other ranges. The shot's
+// StepArgs are read through the constant address space (scalar loads).
+typedef const StepArgs __attribute__((address_space(4))) *cargs_ptr;
+
+template <int T, int C, int D, int MED, bool EDGE>
+__global__ void __launch_bounds__(64 * kWavesPerWG, EDGE ? 4 : 5)
+    pool_kernel(const StepArgs *__restrict__ descs, PoolQueue *q, int k, int tp) {
+  __shared__ float4 lut_s[MED == kTable ? 256 : 1];
+  __shared__ int s_pos;
+  constexpr int cls = EDGE ? 1 : 0;
+  const int4 *items = q->items[cls];
+  const int count = __builtin_amdgcn_readfirstlane(q->count[cls]);
+  const int g = __builtin_amdgcn_readfirstlane(q->g) + k;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int x = blockIdx.x & 7, tried = 0;  // used by thread 0 only
+  int lut_slot = -1;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int pos = -1;
+      while (tried < 8) {
+        const int lo = static_cast<int>(static_cast<long long>(x) * count / 8);
+        const int hi = static_cast<int>(static_cast<long long>(x + 1) * count / 8);
+        const int t = atomicAdd(&q->ticket[k][cls][x], 1);
+        if (lo + t < hi) {
+          pos = lo + t;
+          break;
+        }
+        x = (x + 1) & 7;
+        ++tried;
+      }
+      s_pos = pos;
+    }
+    __syncthreads();
+    const int pos = __builtin_amdgcn_readfirstlane(s_pos);
+    if (pos < 0) return;  // every range is exhausted (uniform over the work-group)
+    const long long e = 4LL * (4LL * pos + wave);
+    const int slot = cld_i(items, 4LL * 4LL * pos + 3) >> 8;
+    const StepArgs &a = *(const StepArgs *)((cargs_ptr)(descs + slot));
+    if constexpr (MED == kTable) {
+      if (slot != lut_slot) {
+        for (int m = threadIdx.x; m < a.nmat; m += blockDim.x) lut_s[m] = gload4(a.lut, m);
+        lut_slot = slot;
+      }
+      __syncthreads();
+    }
+    const int tx = cld_i(items, e), r0 = cld_i(items, e + 1), r1 = cld_i(items, e + 2), w = cld_i(items, e + 3);
+    const int dg = g - a.g_base;
+    const int n0 = a.n_base + dg * tp;
+    const int src = a.src_base ^ (dg & 1);
+    if (r0 < r1) {
+      if (EDGE || (w & 1))
+        run_tile<T, C, D, EDGE, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+      else
+        run_tile<T, C, D, EDGE, MED, false>(a, lut_s, src, n0, tx, r0, r1);
+    }
+    // every wave is done with s_pos and the LUT before they are rewritten
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(64) pool_tick(PoolQueue *q, int nk, int dg) {
+  int *t = &q->ticket[0][0][0];
+  for (int i = threadIdx.x; i < nk * 16; i += blockDim.x) t[i] = 0;
+  if (threadIdx.x == 0) q->g += dg;
+}
+
+__global__ void __launch_bounds__(64) pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0,
+                                               int count1, int g) {
+  if (threadIdx.x == 0) {
+    q->items[0] = items0;
+    q->items[1] = items1;
+    q->count[0] = count0;
+    q->count[1] = count1;
+    if (g >= 0) q->g = g;
+  }
+}
+
+template <int T, int D, bool EDGE>
+hipError_t launch_pool_t(const StepArgs *descs, PoolQueue *q, int k, int tp, int med, int grid, hipStream_t s) {
+  if (med == kTable)
+    hipLaunchKernelGGL((pool_kernel<T, 1, D, kTable, EDGE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q, k,
+                       tp);
+  else
+    hipLaunchKernelGGL((pool_kernel<T, 1, D, kDensity, EDGE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q,
+                       k, tp);
+  return hipGetLastError();
+}
+
+template <int D, bool EDGE>
+hipError_t launch_pool_d(const StepArgs *descs, PoolQueue *q, int k, int tp, int T, int med, int grid,
+                         hipStream_t s) {
+  switch (T) {
+    case 1: return launch_pool_t<1, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 2: return launch_pool_t<2, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 3: return launch_pool_t<3, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 4: return launch_pool_t<4, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 5: return launch_pool_t<5, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 6: return launch_pool_t<6, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 7: return launch_pool_t<7, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 8: return launch_pool_t<8, D, EDGE>(descs, q, k, tp, med, grid, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int T, int C, int D, bool EDGE>
@@ -625,6 +751,26 @@ hipError_t launch_cd(const StepArgs &a, int T, int src, int n0, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D, int medium_kind,
+                       int grid, hipStream_t stream) {
+  if (grid <= 0 || k < 0 || k >= kMaxChunk) return hipErrorInvalidValue;
+  if (cls != 0) return launch_pool_d<1, true>(descs, q, k, tp, T, medium_kind, grid, stream);
+  if (D == 2) return launch_pool_d<2, false>(descs, q, k, tp, T, medium_kind, grid, stream);
+  return launch_pool_d<1, false>(descs, q, k, tp, T, medium_kind, grid, stream);
+}
+
+hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(pool_set, dim3(1), dim3(64), 0, stream, q, items0, items1, count0, count1, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_tick(PoolQueue *q, int nk, int dg, hipStream_t stream) {
+  if (nk < 0 || nk > kMaxChunk) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pool_tick, dim3(1), dim3(64), 0, stream, q, nk, dg);
+  return hipGetLastError();
+}
 
 hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream) {
   // Edge work-groups always use the shallow prefetch (their state already

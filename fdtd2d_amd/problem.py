@@ -343,3 +343,90 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Pool:
+    """Many independent shots advanced together (fdtd2d_pool_*).
+
+    Shots may differ in grid size, medium, sources, detectors and step count
+    (one medium kind per pool). Every shot's results are bitwise those of a
+    Plan of the same problem."""
+
+    def __init__(self, device: int = 0, **options):
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_pool_create(device, C.byref(h)), "fdtd2d_pool_create")
+        self._h = h
+        self._shots = {}
+        if "budget_bytes" in options:  # must precede any shot
+            self.set_option("budget_bytes", options.pop("budget_bytes"))
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def set_option(self, name: str, value: float) -> None:
+        L.check(L.lib().fdtd2d_pool_set_option(self._h, name.encode(), float(value)), "fdtd2d_pool_set_option")
+
+    def info(self, name: str) -> float:
+        v = C.c_double()
+        L.check(L.lib().fdtd2d_pool_get_info(self._h, name.encode(), C.byref(v)), "fdtd2d_pool_get_info")
+        return v.value
+
+    def add(self, problem) -> int:
+        """Copies the problem (a Setup or a Problem) into the pool; returns the shot id."""
+        if isinstance(problem, Setup):
+            ptr = problem.c_problem
+            s = ptr.contents
+        else:
+            s, keep = problem.to_c()
+            ptr = C.byref(s)
+        sid = C.c_int()
+        L.check(L.lib().fdtd2d_pool_add(self._h, ptr, C.byref(sid)), "fdtd2d_pool_add")
+        self._shots[sid.value] = (s.nx, s.ny, s.nsteps, s.ndet, s.ncomp)
+        return sid.value
+
+    def step(self, nlaunch: int) -> None:
+        L.check(L.lib().fdtd2d_pool_step(self._h, nlaunch), "fdtd2d_pool_step")
+
+    def sync(self) -> None:
+        L.check(L.lib().fdtd2d_pool_sync(self._h), "fdtd2d_pool_sync")
+
+    def state(self, shot: int):
+        """(steps done, steps total, complete on the device)."""
+        d, t, c = C.c_int(), C.c_int(), C.c_int()
+        L.check(L.lib().fdtd2d_pool_shot_state(self._h, shot, C.byref(d), C.byref(t), C.byref(c)),
+                "fdtd2d_pool_shot_state")
+        return d.value, t.value, bool(c.value)
+
+    def read_detectors(self, shot: int):
+        nx, ny, nsteps, ndet, ncomp = self._shots[shot]
+        det = np.zeros((nsteps + 1, ndet), np.float32)
+        comp = np.zeros((3, nsteps + 1, ncomp), np.float32)
+        L.check(L.lib().fdtd2d_pool_read_detectors(self._h, shot, _fp(det), det.size, _fp(comp), comp.size),
+                "fdtd2d_pool_read_detectors")
+        return det, comp
+
+    def read_field(self, shot: int, which: int) -> np.ndarray:
+        nx, ny = self._shots[shot][:2]
+        out = np.zeros(FIELD_SHAPES[which](nx, ny), np.float32)
+        L.check(L.lib().fdtd2d_pool_read_field(self._h, shot, which, _fp(out), out.size), "fdtd2d_pool_read_field")
+        return out
+
+    def remove(self, shot: int) -> None:
+        L.check(L.lib().fdtd2d_pool_remove(self._h, shot), "fdtd2d_pool_remove")
+        self._shots.pop(shot, None)
+
+    def close(self) -> None:
+        if self._h:
+            L.lib().fdtd2d_pool_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -223,6 +223,45 @@ int fdtd2d_plan_destroy(fdtd2d_plan *plan);
 int fdtd2d_plan_time_kernels(fdtd2d_plan *plan, int enable);
 int fdtd2d_plan_kernel_ms(fdtd2d_plan *plan, double *avg_ms, int *count);
 
+/* ------------------------------------------------------------------ pools
+ * Many independent shots (any grid size, medium, sources, detectors and step
+ * count; one medium kind per pool) advanced together by persistent kernels
+ * over one packed tile queue: the batched and served forms of the stepper.
+ * No reference counterpart: the reference runs one grid per process
+ * (F/FDTD_cuda.cu:384-441). Results of every shot are bitwise those of a
+ * single plan of the same problem.
+ *
+ * A pool launch advances every unfinished shot by min(tblock, steps left);
+ * fdtd2d_pool_step queues `nlaunch` of them (asynchronously; chunks of
+ * `graph_chunk` launches are replayed from a captured hipGraph when no shot
+ * ends inside them). Shots can be added and removed between steps.
+ *
+ * Options (fdtd2d_pool_set_option): "tblock" (1..8, default 8), "band",
+ * "band_cpml", "prefetch" (1 or 2), "wg_per_cu" / "wg_per_cu_cpml"
+ * (persistent work-groups per CU of each kernel), "graph_chunk" (0..16, 0 =
+ * no graphs), "budget_bytes" (before any shot: carve shots from one arena of
+ * that size and refuse shots that do not fit, FDTD2D_ENOMEM).
+ * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",
+ * "shots", "running", "launches", "cell_updates", "graph_replays",
+ * "eager_launches", "tail_launches", "cus". */
+typedef struct fdtd2d_pool fdtd2d_pool;
+int fdtd2d_pool_create(int device, fdtd2d_pool **out);
+int fdtd2d_pool_set_option(fdtd2d_pool *pool, const char *name, double value);
+int fdtd2d_pool_get_info(const fdtd2d_pool *pool, const char *name, double *value);
+/* Copies the problem into the pool; *shot = its id (ids of removed shots are reused). */
+int fdtd2d_pool_add(fdtd2d_pool *pool, const fdtd2d_problem *p, int *shot);
+int fdtd2d_pool_step(fdtd2d_pool *pool, int nlaunch);
+int fdtd2d_pool_sync(fdtd2d_pool *pool);
+/* complete = all steps issued and finished on the device (does not block). */
+int fdtd2d_pool_shot_state(fdtd2d_pool *pool, int shot, int *steps_done, int *steps_total, int *complete);
+/* As fdtd2d_read_detectors / fdtd2d_read_field for one shot; waits only for
+   the launches the shot's results depend on. */
+int fdtd2d_pool_read_detectors(fdtd2d_pool *pool, int shot, float *det, size_t det_count, float *comp,
+                               size_t comp_count);
+int fdtd2d_pool_read_field(fdtd2d_pool *pool, int shot, int which, float *out, size_t count);
+int fdtd2d_pool_remove(fdtd2d_pool *pool, int shot);
+int fdtd2d_pool_destroy(fdtd2d_pool *pool);
+
 #ifdef __cplusplus
 }
 #endif
