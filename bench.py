@@ -135,6 +135,140 @@ def stats_summary(path: str) -> int:
     return 0
 
 
+def bench_batch(args) -> int:
+    """The 256-shot uneven mix (seed 3): every shot through one pool (persistent
+    kernels, graph chunks, autotuned shape) against looping the plain
+    single-grid stepper over the same shots, with the batched detector traces
+    checked bitwise against the plain ones."""
+    import numpy as np
+
+    import fdtd2d_amd as F
+    from fdtd2d_amd.batch import build_setups, run_pool
+
+    t0 = time.time()
+    setups = build_setups(3, 3, range(args.shots))
+    probs = [s.c_problem.contents for s in setups]
+    cells = sum(float(p.nx) * p.ny * p.nsteps for p in probs)
+    log(f"batch: {len(setups)} shots, {cells / 1e12:.3f} T cell updates, set up in {time.time() - t0:.1f}s")
+
+    pool, ids, st = run_pool(setups, 0, tune=not args.no_tune, log=log)
+    det_b = [pool.read_detectors(i)[0] for i in ids]
+    pool.close()
+    tb = st["seconds"]
+    log(f"batched: {tb:.3f}s, {cells / tb / 1e9:.1f} Gcell/s, shape {st['tuned']['shape'] if st['tuned'] else None}, "
+        f"graph replays {st['graph_replays']:.0f}, eager {st['eager_launches']:.0f}, tails {st['tail_launches']:.0f}")
+
+    tp = None
+    same = None
+    if not args.no_plain:
+        tp = 0.0
+        same = True
+        nonfinite = 0
+        for k, s in enumerate(setups):
+            with F.Plan(s, 0) as plan:
+                plan.sync()
+                c0 = time.perf_counter()
+                plan.run(probs[k].nsteps)
+                plan.sync()
+                tp += time.perf_counter() - c0
+                d, _ = plan.read_detectors()
+            # bit patterns, so that equal NaNs compare equal too
+            same = same and bool(np.array_equal(d.view(np.uint32), det_b[k].view(np.uint32)))
+            nonfinite += int((~np.isfinite(d)).sum())
+        log(f"plain loop: {tp:.3f}s, {cells / tp / 1e9:.1f} Gcell/s; batched traces bitwise equal: {same}; "
+            f"non-finite trace values: {nonfinite}")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Oracle
+
+        ssum, csum = 0.0, 0.0
+        for s in setups[:2]:
+            ref = Oracle(s, precision=32, threads=1)
+            c0 = time.perf_counter()
+            ref.run(16)
+            ssum += time.perf_counter() - c0
+            csum += float(s.c_problem.contents.nx) * s.c_problem.contents.ny * 16
+            ref.close()
+        rate = csum / ssum
+        cpu = {"value": round(rate / (cells / len(setups)), 6), "unit": "sims/s", "cores": 1, "kind": "port",
+               "sample": f"FP32 CPU oracle, shots 0-1 of the mix x 16 steps ({ssum:.1f} s, single thread, "
+                         f"{rate / 1e9:.3f} Gcell/s), scaled to whole shots"}
+    line = {
+        "metric": "batched sims/s",
+        "value": round(len(setups) / tb, 3),
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(setups)} uneven shots (seed 3): 256-2048 per side, 8-40 GHz, 1-16 elements, "
+                               "linear/parabolic plasma, 0-5% fluctuation, CPML 12, 4000 steps",
+                   "cell_updates": cells, "parallelism": "single GPU"},
+        "batched": {"seconds": round(tb, 4), "gcell_per_s": round(cells / tb / 1e9, 2),
+                    "shape": st["tuned"]["shape"] if st["tuned"] else None,
+                    "graph_replays": st["graph_replays"], "eager_launches": st["eager_launches"]},
+        "plain": None if tp is None else {"seconds": round(tp, 4), "sims_per_s": round(len(setups) / tp, 3),
+                                          "gcell_per_s": round(cells / tp / 1e9, 2)},
+        "speedup_vs_plain": None if tp is None else round(tp / tb, 3),
+        "traces_bitwise_equal_to_plain": same,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if same is not False else 1
+
+
+def bench_serve(args) -> int:
+    """Served streams through the continuous-batching loop: frequency sweeps
+    (--mode serve-sweep) or the mixed 1024^2 / 16384^2 stream (--mode
+    serve-mixed), at --rate requests/s for --duration seconds of arrivals."""
+    from fdtd2d_amd.serve import Server, latency_summary, mixed_requests, sweep_requests
+
+    t0 = time.time()
+    if args.mode == "serve-sweep":
+        rate = args.rate or 200.0
+        reqs = sweep_requests(rate, args.duration, seed=4)
+        budget = int(160e9)
+        classes = ["sweep"]
+    else:
+        rate = args.rate or 0.3
+        reqs = mixed_requests(rate, args.duration, seed=5)
+        budget = int(240e9)
+        classes = ["1024", "16384"]
+    nshots = sum(len(r.shots) for r in reqs)
+    log(f"{args.mode}: {len(reqs)} requests ({nshots} shots) at {rate} req/s over {args.duration}s; "
+        f"drawn in {time.time() - t0:.1f}s")
+    srv = Server(0, budget_bytes=budget, tblock=8, band=128, wg_per_cu=3, wg_per_cu_cpml=2, prefetch=1)
+    done, st = srv.run(reqs, max_seconds=args.duration + args.drain, log=log)
+    srv.close()
+    lat = latency_summary(done, classes)
+    wall = st["wall"]
+    line = {
+        "metric": "p50/p99 request latency",
+        "value": None if lat["all"] is None else round(lat["all"]["p50"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.mode, "rate_req_per_s": rate, "arrival_window_s": args.duration,
+                   "requests": len(reqs), "shots": nshots, "budget_bytes": budget, "parallelism": "single GPU"},
+        "latency_s": lat,
+        "completed_requests": len(done),
+        "unfinished_requests": st["unfinished_requests"],
+        "sims_per_s": round(st["shots"] / wall, 3),
+        "gcell_per_s": round(st["cell_updates"] / wall / 1e9, 2),
+        "hbm_gbs_algorithmic": round(36 * st["cell_updates"] / wall / 1e9, 1),
+        "wall_s": round(wall, 2),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
@@ -149,6 +283,13 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_8192.json"))
+    ap.add_argument("--mode", choices=["single", "batch", "serve-sweep", "serve-mixed"], default="single")
+    ap.add_argument("--rate", type=float, default=0.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--drain", type=float, default=120.0)
+    ap.add_argument("--shots", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
@@ -160,8 +301,13 @@ def main() -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
-              "(shots shard across GPUs by construction, see the design notes)", file=sys.stderr)
+              "(a single grid would shard in row slabs with halo exchange, batched and served "
+              "shots by shot; neither is built)", file=sys.stderr)
         return 2
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode.startswith("serve"):
+        return bench_serve(args)
 
     import fdtd2d_amd as F
 

@@ -124,6 +124,7 @@ SIGNATURES = [
     ("fdtd2d_pool_add", C.c_int, [C.c_void_p, C.POINTER(ProblemC), C.POINTER(C.c_int)]),
     ("fdtd2d_pool_step", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_pool_sync", C.c_int, [C.c_void_p]),
+    ("fdtd2d_pool_wait", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_pool_shot_state", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int)]),
     ("fdtd2d_pool_read_detectors", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t, _f, C.c_size_t]),

@@ -1,0 +1,87 @@
+"""Batched shots: building a shot mix, running it through one pool, and the
+launch-shape autotuner.
+
+The autotuner tunes a live pool by coordinate descent over the launch shape
+(steps per launch, band rows, CPML sub-band rows, persistent work-groups per
+CU of each kernel, prefetch depth). Every trial advances the real shots -
+results are bitwise independent of the shape - and is scored by the cell
+updates it completed per second of wall time, measured with the pool synced
+on both sides.
+"""
+
+from __future__ import annotations
+
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+from .problem import Pool, Setup
+
+# coordinate-descent axes, in the order they are tuned; the first value of
+# each is the starting point
+TUNE_AXES = [
+    ("tblock", [8, 6, 7]),
+    ("band", [128, 64, 256]),
+    ("wg_per_cu", [4, 3, 5]),
+    ("band_cpml", [32, 16, 64]),
+    ("wg_per_cu_cpml", [1, 2]),
+    ("prefetch", [2, 1]),
+]
+
+
+def build_setups(workload: int, seed: int, shots, threads: int = 16, steps: int = 0):
+    """Setups of the given shots of a workload, built on host threads (the
+    builders run in C++ and release the interpreter lock)."""
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(lambda s: Setup.workload(workload, seed=seed, shot=s, steps=steps), shots))
+
+
+def _trial(pool: Pool, launches: int) -> float:
+    pool.sync()
+    c0 = pool.info("cell_updates")
+    t0 = time.perf_counter()
+    pool.step(launches)
+    pool.sync()
+    dt = time.perf_counter() - t0
+    return (pool.info("cell_updates") - c0) / dt if dt > 0 else 0.0
+
+
+def autotune(pool: Pool, launches_per_trial: int = 4, axes=TUNE_AXES, log=None) -> dict:
+    """Picks the launch shape for the shots now in the pool; returns it with
+    the score of every trial. Stops early when the pool runs out of work."""
+    best = {name: values[0] for name, values in axes}
+    for name, value in best.items():
+        pool.set_option(name, value)
+    trials = []
+    best_rate = _trial(pool, launches_per_trial)
+    trials.append((dict(best), best_rate))
+    for name, values in axes:
+        for v in values[1:]:
+            if pool.info("running") == 0:
+                break
+            pool.set_option(name, v)
+            rate = _trial(pool, launches_per_trial)
+            trials.append(({**best, name: v}, rate))
+            if log:
+                log(f"autotune {name}={v}: {rate / 1e9:.1f} Gcell/s (best {best_rate / 1e9:.1f})")
+            if rate > best_rate:
+                best_rate, best[name] = rate, v
+        pool.set_option(name, best[name])
+    return {"shape": best, "rate": best_rate, "trials": trials}
+
+
+def run_pool(setups, device: int = 0, tune: bool = True, chunk_launches: int = 8, log=None, **options):
+    """All setups through one pool to completion. Returns (pool, ids, stats);
+    the caller reads results and closes the pool."""
+    pool = Pool(device, **options)
+    ids = [pool.add(s) for s in setups]
+    pool.sync()
+    t0 = time.perf_counter()
+    tuned = autotune(pool, log=log) if tune else None
+    while pool.info("running") > 0:
+        pool.step(chunk_launches)
+    pool.sync()
+    dt = time.perf_counter() - t0
+    stats = {"seconds": dt, "cell_updates": pool.info("cell_updates"), "tuned": tuned,
+             "launches": pool.info("launches"), "graph_replays": pool.info("graph_replays"),
+             "eager_launches": pool.info("eager_launches"), "tail_launches": pool.info("tail_launches")}
+    return pool, ids, stats

@@ -633,6 +633,14 @@ int fdtd2d_pool_sync(fdtd2d_pool *pp) {
   return FDTD2D_OK;
 }
 
+int fdtd2d_pool_wait(fdtd2d_pool *pp, int max_pending) {
+  if (!pp) return set_error(FDTD2D_EINVAL, "null pool");
+  if (max_pending < 0) return set_error(FDTD2D_EINVAL, "negative launch count");
+  HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+  retire(pp, pp->g - max_pending);
+  return FDTD2D_OK;
+}
+
 int fdtd2d_pool_shot_state(fdtd2d_pool *pp, int id, int *steps_done, int *steps_total, int *complete) {
   Shot *sh = get_shot(pp, id);
   if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");

@@ -208,6 +208,12 @@ class Setup:
         L.check(L.lib().fdtd2d_setup_get(self._h, name.encode(), C.byref(v)), f"fdtd2d_setup_get({name})")
         return v.value
 
+    def footprint(self) -> int:
+        """Device bytes a plan (or pool shot) of this setup holds."""
+        out = C.c_uint64()
+        L.check(L.lib().fdtd2d_plan_footprint(self.c_problem, C.byref(out)), "fdtd2d_plan_footprint")
+        return int(out.value)
+
     def write_outputs(self, out_dir: str, det: np.ndarray, comp: np.ndarray, seconds: float = 0.0) -> None:
         d = np.ascontiguousarray(det, np.float32)
         c = np.ascontiguousarray(comp, np.float32)
@@ -388,6 +394,10 @@ class Pool:
 
     def sync(self) -> None:
         L.check(L.lib().fdtd2d_pool_sync(self._h), "fdtd2d_pool_sync")
+
+    def wait(self, max_pending: int) -> None:
+        """Blocks until at most max_pending issued launches are unfinished."""
+        L.check(L.lib().fdtd2d_pool_wait(self._h, max_pending), "fdtd2d_pool_wait")
 
     def state(self, shot: int):
         """(steps done, steps total, complete on the device)."""

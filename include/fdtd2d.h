@@ -145,7 +145,9 @@ int fdtd2d_ini_lookup(const char *text, const char *section, const char *name, i
  *   3  one shot of the 256-shot uneven batch mix (shot = 0..255)
  *   4  one frequency of a served sweep request (shot = request*64 + frequency index)
  *   5  one shot of the mixed-size serving stream (shot = request index)
- * steps_override > 0 replaces the workload's step count. */
+ * steps_override > 0 replaces the workload's step count; for workloads 3-5,
+ * steps_override < 0 only makes the random draws (fdtd2d_setup_get of
+ * "Imax", "Jmax", "nMax", "nfreq", "big", ...) and builds no problem. */
 int fdtd2d_setup_workload(int workload, uint64_t seed, int shot, int steps_override,
                           fdtd2d_setup **out);
 
@@ -252,6 +254,9 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pool, const char *name, double *valu
 int fdtd2d_pool_add(fdtd2d_pool *pool, const fdtd2d_problem *p, int *shot);
 int fdtd2d_pool_step(fdtd2d_pool *pool, int nlaunch);
 int fdtd2d_pool_sync(fdtd2d_pool *pool);
+/* Blocks until at most max_pending of the launches issued so far are still
+   unfinished on the device (keeps a serving loop a bounded distance ahead). */
+int fdtd2d_pool_wait(fdtd2d_pool *pool, int max_pending);
 /* complete = all steps issued and finished on the device (does not block). */
 int fdtd2d_pool_shot_state(fdtd2d_pool *pool, int shot, int *steps_done, int *steps_total, int *complete);
 /* As fdtd2d_read_detectors / fdtd2d_read_field for one shot; waits only for

@@ -229,3 +229,21 @@ def test_gaussian_field_statistics():
     lag = 20
     r = np.mean(g[:, :-lag] * g[:, lag:]) / g.var()
     assert r == pytest.approx(math.exp(-1), abs=0.05)
+
+
+def test_synthetic_shots_respect_the_yee_drude_stability_bound():
+    """Shots of the batched and served workloads (dx = lambda/20, plasma up to
+    3x critical) take dt under (c dt)^2 (1/dx^2 + 1/dy^2) + (wp dt/2)^2 <= 1
+    for their densest cell; shot 176 of the batch mix (2.7x critical at
+    22.7 GHz) diverged at 0.99 of the plain Courant limit."""
+    import numpy as np
+
+    c = 299792458.0
+    for w, shot in [(3, 176), (3, 0), (3, 1), (4, 0), (5, 0)]:
+        s = F.Setup.workload(w, seed=w, shot=shot, steps=10)
+        p = s.c_problem.contents
+        dens = np.ctypeslib.as_array(p.density, shape=(p.ny * p.nx,))
+        wp2 = float(dens.max()) * 2.81794e-8 / EPS0
+        bound = (c * p.dt) ** 2 * (1 / p.dx ** 2 + 1 / p.dy ** 2) + wp2 * p.dt ** 2 / 4
+        assert bound <= 1.0, (w, shot, bound)
+        s.close()
