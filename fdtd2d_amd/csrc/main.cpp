@@ -56,8 +56,8 @@ int main(int argc, char **argv) {
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::printf("Done time-stepping...\n");
   if ((rc = fdtd2d_write_outputs(s, out.c_str(), det.data(), comp.data(), secs))) return die(rc, "write_outputs");
-  fdtd2d_plan_destroy(plan);
-  fdtd2d_setup_destroy(s);
   std::printf("%d steps of %dx%d in %.3f s\n", n, p->nx, p->ny, secs);
+  fdtd2d_plan_destroy(plan);
+  fdtd2d_setup_destroy(s);  // p points into s
   return 0;
 }
