@@ -86,7 +86,7 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
   L.off_lut = take(256 * 16);
   for (int f = 0; f < 8; ++f) L.off_col[f] = take(static_cast<size_t>(L.pitch) * 4);
   for (int f = 0; f < 8; ++f) L.off_row[f] = take(static_cast<size_t>(L.ny) * 4);
-  L.off_rowinfo = take(static_cast<size_t>(L.ny) * 16);
+  L.off_rowinfo = take(static_cast<size_t>(L.ny) * fdtd2d::kRowRec * 4);
   L.off_evoff = take(static_cast<size_t>(L.ny + 1) * 4);
   L.off_events = take(n_events * sizeof(fdtd2d::Event));
   L.off_src = take(static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4);
@@ -523,6 +523,29 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       if ((rc = upload(pl, L.off_row[f], row[f].data(), ny * 4))) return fail(rc);
     if ((rc = upload(pl, L.off_row[6], hxy_s.data(), ny * 4))) return fail(rc);
     if ((rc = upload(pl, L.off_row[7], ezy_s.data(), ny * 4))) return fail(rc);
+    // the per-row record the CPML-class kernel reads at each level
+    std::vector<int32_t> ri(static_cast<size_t>(ny) * fdtd2d::kRowRec, 0);
+    auto bits = [](float v) {
+      int32_t b;
+      std::memcpy(&b, &v, 4);
+      return b;
+    };
+    for (int j = 0; j < ny; ++j) {
+      const int jc = std::min(j, ny - 2);
+      int32_t *q = &ri[static_cast<size_t>(j) * fdtd2d::kRowRec];
+      q[0] = bits(p->den_hy[jc]);
+      q[1] = bits(p->den_ey[jc]);
+      q[2] = ev_off[j] < ev_off[j + 1] ? ev_off[j] : -1;
+      q[3] = ev_off[j + 1];
+      const bool hrow = j <= ny - 2;  // psi updates act on rows 0..ny-2 only
+      q[4] = hrow ? hxy_s[j] : -1;
+      q[5] = hrow ? ezy_s[j] : -1;
+      q[6] = bits(hrow ? row[2][j] : 0.f);
+      q[7] = bits(hrow ? row[3][j] : 0.f);
+      q[8] = bits(hrow ? row[4][j] : 0.f);
+      q[9] = bits(hrow ? row[5][j] : 0.f);
+    }
+    if ((rc = upload(pl, L.off_rowinfo, ri.data(), ri.size() * 4))) return fail(rc);
   }
   // uniform interior den values (bitwise) and the rows / columns that differ
   float u_hx, u_ex, u_hy, u_ey;
@@ -566,17 +589,6 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       }
     }
   if ((rc = upload(pl, L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
-  {
-    std::vector<int4> ri(ny);
-    for (int j = 0; j < ny; ++j) {
-      const int jc = std::min(j, ny - 2);
-      int dh, de;
-      std::memcpy(&dh, &p->den_hy[jc], 4);
-      std::memcpy(&de, &p->den_ey[jc], 4);
-      ri[j] = make_int4(dh, de, ev_off[j] < ev_off[j + 1] ? ev_off[j] : -1, ev_off[j + 1]);
-    }
-    if ((rc = upload(pl, L.off_rowinfo, ri.data(), ri.size() * sizeof(int4)))) return fail(rc);
-  }
   if ((rc = upload(pl, L.off_events, events.data(), events.size() * sizeof(fdtd2d::Event)))) return fail(rc);
   if ((rc = upload(pl, L.off_src, p->src_table, static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4)))
     return fail(rc);
@@ -634,7 +646,7 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   a.hxy_slot = reinterpret_cast<const int32_t *>(pl->base + L.off_row[6]);
   a.ezy_slot = reinterpret_cast<const int32_t *>(pl->base + L.off_row[7]);
   a.ev_off = reinterpret_cast<const int32_t *>(pl->base + L.off_evoff);
-  a.rowinfo = reinterpret_cast<const int4 *>(pl->base + L.off_rowinfo);
+  a.rowinfo = reinterpret_cast<const int *>(pl->base + L.off_rowinfo);
   a.events = reinterpret_cast<const fdtd2d::Event *>(pl->base + L.off_events);
   a.src_table = reinterpret_cast<const float *>(pl->base + L.off_src);
   a.nsrc = p->nsrc;

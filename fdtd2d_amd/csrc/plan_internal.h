@@ -48,9 +48,10 @@ struct StepArgs {
   const int32_t *hyx_slot, *ezx_slot;          // [pitch], -1 outside the strips
   const float *hxy_b, *hxy_c, *ezy_b, *ezy_c;  // [ny]
   const int32_t *hxy_slot, *ezy_slot;          // [ny]
-  // per row: {den_hy bits, den_ey bits, first event or -1 when the row has
-  // none, end event}, [ny]
-  const int4 *rowinfo;
+  // per row, kRowRec ints: {den_hy bits, den_ey bits, first event or -1 when
+  // the row has none, end event, psi_hxy slot, psi_ezy slot (-1 outside the
+  // y strips), hxy b, hxy c, ezy b, ezy c (bits), 0, 0}, [ny]
+  const int *rowinfo;
   // events by row
   const int32_t *ev_off;  // [ny+1]
   const Event *events;
@@ -98,6 +99,8 @@ struct PoolQueue {
   int pad;
   int ticket[kMaxChunk][2][8];
 };
+
+constexpr int kRowRec = 12;  // ints per row record
 
 // One launch advances T steps (1..kMaxT).
 constexpr int kMaxT = 8;

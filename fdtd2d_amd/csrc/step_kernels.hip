@@ -106,8 +106,9 @@ struct Cols {
   int hyx_s[EDGE ? C : 1], ezx_s[EDGE ? C : 1];
 };
 
-// Per-row scalars, loaded once when the row enters the pipeline and carried
-// down the levels with it (they are uniform across the wave).
+// Per-row scalars of the row a level works on (uniform across the wave):
+// one 48-byte record per row, read by each level when it reaches the row
+// (carrying them down the levels instead would hold T+1 records in SGPRs).
 template <bool EDGE>
 struct RowS {
   float den_hy, den_ey;
@@ -129,22 +130,19 @@ __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
     return q;
   }
   const bool in = r >= 0 && r < a.ny;
-  const long long ri = 4LL * (in ? r : 0);
+  const long long ri = static_cast<long long>(kRowRec) * (in ? r : 0);
   q.den_hy = __int_as_float(cld_i(a.rowinfo, ri));
   q.den_ey = __int_as_float(cld_i(a.rowinfo, ri + 1));
   const int e0 = cld_i(a.rowinfo, ri + 2), e1 = cld_i(a.rowinfo, ri + 3);
   q.e0 = in && e0 >= 0 ? e0 : 0;
   q.e1 = in && e0 >= 0 ? e1 : 0;
-  {
-    if (r >= 0 && r <= a.ny - 2) {
-      q.hxy_s = cld(a.hxy_slot, r);
-      q.ezy_s = cld(a.ezy_slot, r);
-      q.hxy_b = cld(a.hxy_b, r);
-      q.hxy_c = cld(a.hxy_c, r);
-      q.ezy_b = cld(a.ezy_b, r);
-      q.ezy_c = cld(a.ezy_c, r);
-    }
-  }
+  const int hs = cld_i(a.rowinfo, ri + 4), es = cld_i(a.rowinfo, ri + 5);
+  q.hxy_s = in ? hs : -1;
+  q.ezy_s = in ? es : -1;
+  q.hxy_b = __int_as_float(cld_i(a.rowinfo, ri + 6));
+  q.hxy_c = __int_as_float(cld_i(a.rowinfo, ri + 7));
+  q.ezy_b = __int_as_float(cld_i(a.rowinfo, ri + 8));
+  q.ezy_c = __int_as_float(cld_i(a.rowinfo, ri + 9));
   return q;
 }
 
@@ -436,7 +434,7 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
 template <int T, int C, bool EDGE, int MED, bool JZ, bool EV>
 __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl, int n0,
                                        int k, int jb0, int jb1, int own0, int own1, Level<C, EDGE> prev,
-                                       Level<C, EDGE> (&st)[T + 1], const RowS<EDGE> (&rs)[T + 1]) {
+                                       Level<C, EDGE> (&st)[T + 1]) {
   float nxt_ez[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) nxt_ez[c] = st[0].ez[c];
@@ -445,8 +443,8 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
     const int r = k - t;
     Level<C, EDGE> out;
     const bool own_row = r >= jb0 && r < jb1;
-    advance<C, EDGE, MED, JZ, EV>(a, lut_s, cl, rs[t], prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row,
-                                  out);
+    const RowS<EDGE> rs = row_scalars<EDGE>(a, r);
+    advance<C, EDGE, MED, JZ, EV>(a, lut_s, cl, rs, prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
     prev = st[t];
     st[t] = out;
 #pragma unroll
@@ -460,15 +458,11 @@ template <int T, int C, bool EDGE, int MED, bool JZ>
 __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, const float4 *lut_s,
                                          const Cols<C, EDGE> &cl, int src, int n0, int k, int jb0, int jb1,
                                          int own0, int own1, bool lane_owns, unsigned trash_off,
-                                         const Level<C, EDGE> &in, Level<C, EDGE> (&st)[T + 1],
-                                         RowS<EDGE> (&rs)[T + 1]) {
-#pragma unroll
-  for (int t = T; t >= 1; --t) rs[t] = rs[t - 1];
-  rs[0] = row_scalars<EDGE>(a, k);
+                                         const Level<C, EDGE> &in, Level<C, EDGE> (&st)[T + 1]) {
   Level<C, EDGE> prev = st[0];
   st[0] = in;
   // ---- levels 1..T, each one row behind the previous
-  levels<T, C, EDGE, MED, JZ, EDGE>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st, rs);
+  levels<T, C, EDGE, MED, JZ, EDGE>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st);
   // ---- store step n0+T for owned cells of row k-T
   const int r = k - T;
   if constexpr (!EDGE) {
@@ -492,7 +486,7 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
       store_row<C>(a.hx[1 - src], rowp, cl.c0, own0, own1, o.hx);
       store_row<C>(a.hy[1 - src], rowp, cl.c0, own0, own1, o.hy);
       store_row<C>(a.jz[1 - src], rowp, cl.c0, own0, own1, o.jz);
-      const int sh = rs[T].hxy_s, se = rs[T].ezy_s;
+      const int sh = cld(a.hxy_slot, r), se = cld(a.ezy_slot, r);
       if (sh >= 0) store_row<C>(a.psi_hxy[1 - src], static_cast<long long>(sh) * pitch, cl.c0, own0, own1, o.phxy);
       if (se >= 0) store_row<C>(a.psi_ezy[1 - src], static_cast<long long>(se) * pitch, cl.c0, own0, own1, o.pezy);
       const long long rowx = static_cast<long long>(r) * a.xs;
@@ -551,10 +545,8 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
   dst.jz = (const gchar *)a.jz[1 - src];
   dst.med = sm.med;
   Level<C, EDGE> st[T + 1];
-  RowS<EDGE> rs[T + 1];
 #pragma unroll
   for (int t = 0; t <= T; ++t) {
-    rs[t] = row_scalars<EDGE>(a, -1);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       st[t].ez[c] = st[t].hx[c] = st[t].hy[c] = st[t].jz[c] = 0.f;
@@ -586,7 +578,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       const int kk = EDGE ? kn : min(kn, jl1 - 1);
       load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
       row_step<T, C, EDGE, MED, JZ>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns, trash_off, in,
-                                st, rs);
+                                    st);
     }
   }
 }
