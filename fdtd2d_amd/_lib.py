@@ -70,6 +70,12 @@ class ProblemC(C.Structure):
         ("ncomp", C.c_int32),
         ("comp_i", _i),
         ("comp_j", _i),
+        ("nrc", C.c_int32),
+        ("rc_i", _i),
+        ("rc_imax", _i),
+        ("rc_jcut", C.c_int32),
+        ("rc_level", C.c_float),
+        ("rc_edge0", C.c_float),
     ]
 
 
@@ -98,6 +104,8 @@ SIGNATURES = [
     ("fdtd2d_setup_get", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     ("fdtd2d_setup_destroy", None, [C.c_void_p]),
     ("fdtd2d_write_outputs", C.c_int, [C.c_void_p, C.c_char_p, _f, _f, C.c_double]),
+    ("fdtd2d_write_rc_outputs", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32)]),
+    ("fdtd2d_read_rc", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_size_t]),
     ("fdtd2d_plan_create", C.c_int, [C.POINTER(ProblemC), C.c_int, C.POINTER(C.c_void_p)]),
     ("fdtd2d_plan_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
     ("fdtd2d_plan_set_tblock", C.c_int, [C.c_void_p, C.c_int]),

@@ -56,6 +56,11 @@ int main(int argc, char **argv) {
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::printf("Done time-stepping...\n");
   if ((rc = fdtd2d_write_outputs(s, out.c_str(), det.data(), comp.data(), secs))) return die(rc, "write_outputs");
+  {
+    std::vector<int32_t> spd(static_cast<size_t>(p->nsteps + 1) * p->nrc);
+    if (p->nrc > 0 && (rc = fdtd2d_read_rc(plan, spd.data(), spd.size()))) return die(rc, "read_rc");
+    if ((rc = fdtd2d_write_rc_outputs(s, out.c_str(), spd.data()))) return die(rc, "write_rc_outputs");
+  }
   std::printf("%d steps of %dx%d in %.3f s\n", n, p->nx, p->ny, secs);
   fdtd2d_plan_destroy(plan);
   fdtd2d_setup_destroy(s);  // p points into s

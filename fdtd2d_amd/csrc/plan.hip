@@ -92,6 +92,20 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
   L.off_src = take(static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4);
   L.off_det = take(static_cast<size_t>(p->nsteps + 1) * p->ndet * 4);
   L.off_comp = take(static_cast<size_t>(3) * (p->nsteps + 1) * p->ncomp * 4);
+  {
+    std::vector<int> cols;
+    for (int e = 0; e < p->nrc; ++e) {
+      cols.push_back(p->rc_i[e]);
+      cols.push_back(p->rc_imax[e]);
+    }
+    std::sort(cols.begin(), cols.end());
+    L.nprobe = static_cast<int>(std::unique(cols.begin(), cols.end()) - cols.begin());
+  }
+  L.off_probe = take(static_cast<size_t>(L.pitch) * 4);
+  L.off_cap = take(static_cast<size_t>(fdtd2d::kMaxT) * L.nprobe * L.ny * 4);
+  L.off_rcmap = take(static_cast<size_t>(2) * p->nrc * 4);
+  L.off_rcedge = take(static_cast<size_t>(p->nrc) * 4);
+  L.off_spd = take(static_cast<size_t>(p->nsteps + 1) * p->nrc * 4);
   L.total = o;
   return L;
 }
@@ -556,9 +570,12 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     u_hy = p->den_hy[jm];
     u_ey = p->den_ey[jm];
     auto same = [](float x, float y) { return std::memcmp(&x, &y, 4) == 0; };
+    // RC probe columns are special too: only the CPML-class kernel captures them
+    std::vector<char> probe(nx, 0);
+    for (int e = 0; e < p->nrc; ++e) probe[p->rc_i[e]] = probe[p->rc_imax[e]] = 1;
     pl->scol_pref.assign(nx + 1, 0);
     for (int i = 0; i < nx; ++i) {
-      const bool sp = i >= nx - 1 || !same(p->den_hx[i], u_hx) || !same(p->den_ex[i], u_ex);
+      const bool sp = i >= nx - 1 || !same(p->den_hx[i], u_hx) || !same(p->den_ex[i], u_ex) || probe[i];
       pl->scol_pref[i + 1] = pl->scol_pref[i] + sp;
     }
     pl->srow_pref.assign(ny + 1, 0);
@@ -590,6 +607,27 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     }
   if ((rc = upload(pl, L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
   if ((rc = upload(pl, L.off_events, events.data(), events.size() * sizeof(fdtd2d::Event)))) return fail(rc);
+  // RC tracker: probe index per column, per-line probe pair, edge levels
+  {
+    std::vector<int32_t> pcol(pitch, -1);
+    std::vector<int> cols;
+    for (int e = 0; e < p->nrc; ++e) {
+      cols.push_back(p->rc_i[e]);
+      cols.push_back(p->rc_imax[e]);
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    for (size_t q = 0; q < cols.size(); ++q) pcol[cols[q]] = static_cast<int32_t>(q);
+    std::vector<int32_t> map(2 * static_cast<size_t>(p->nrc));
+    for (int e = 0; e < p->nrc; ++e) {
+      map[2 * e] = pcol[p->rc_imax[e]];
+      map[2 * e + 1] = pcol[p->rc_i[e]];
+    }
+    std::vector<float> edge(p->nrc, p->rc_edge0);
+    if ((rc = upload(pl, L.off_probe, pcol.data(), pcol.size() * 4))) return fail(rc);
+    if ((rc = upload(pl, L.off_rcmap, map.data(), map.size() * 4))) return fail(rc);
+    if ((rc = upload(pl, L.off_rcedge, edge.data(), edge.size() * 4))) return fail(rc);
+  }
   if ((rc = upload(pl, L.off_src, p->src_table, static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4)))
     return fail(rc);
 
@@ -655,6 +693,19 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   a.comp = reinterpret_cast<float *>(pl->base + L.off_comp);
   a.ncomp = p->ncomp;
   a.comp_plane = static_cast<long long>(p->nsteps + 1) * p->ncomp;
+  a.probe_col = reinterpret_cast<const int *>(pl->base + L.off_probe);
+  a.rc_cap = reinterpret_cast<float *>(pl->base + L.off_cap);
+  a.nprobe = L.nprobe;
+  pl->nrc = p->nrc;
+  pl->rc.cap = a.rc_cap;
+  pl->rc.map = reinterpret_cast<const int *>(pl->base + L.off_rcmap);
+  pl->rc.edge = reinterpret_cast<float *>(pl->base + L.off_rcedge);
+  pl->rc.spd = reinterpret_cast<int *>(pl->base + L.off_spd);
+  pl->rc.nrc = p->nrc;
+  pl->rc.nprobe = L.nprobe;
+  pl->rc.ny = ny;
+  pl->rc.jcut = p->rc_jcut;
+  pl->rc.level = p->rc_level;
   if ((rc = set_tiling(pl))) return fail(rc);
   *out = pl;
   return FDTD2D_OK;
@@ -762,6 +813,10 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       pl->pending.push_back({e0, e1});
     }
     if (pl->a.n_wg[1] > 0) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
+    if (pl->nrc > 0) {
+      e = fdtd2d::launch_rc(pl->rc, pl->n, t, pl->stream);
+      if (e != hipSuccess) return hip_fail(e, "RC kernel launch");
+    }
     pl->launches_T[t] += 1;
     pl->src = 1 - pl->src;
     pl->n += t;
@@ -797,6 +852,16 @@ int fdtd2d_read_detectors(fdtd2d_plan *pl, float *det, size_t det_count, float *
     if (comp_count != nc) return set_error(FDTD2D_EINVAL, "comp_count must be 3*(nsteps+1)*ncomp");
     if (nc) HIP_TRY(hipMemcpy(comp, pl->a.comp, nc * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H comp");
   }
+  return FDTD2D_OK;
+}
+
+int fdtd2d_read_rc(fdtd2d_plan *pl, int32_t *spd, size_t count) {
+  if (!pl || !spd) return set_error(FDTD2D_EINVAL, "null argument");
+  const size_t n = static_cast<size_t>(pl->nsteps + 1) * pl->nrc;
+  if (count != n) return set_error(FDTD2D_EINVAL, "count must be (nsteps+1)*nrc");
+  HIP_TRY(hipSetDevice(pl->device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
+  if (n) HIP_TRY(hipMemcpy(spd, pl->rc.spd, n * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H rc");
   return FDTD2D_OK;
 }
 

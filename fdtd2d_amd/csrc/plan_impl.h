@@ -38,6 +38,8 @@ struct Layout {
   size_t off_col[8];  // den_ex, den_hx, hyx_b, hyx_c, ezx_b, ezx_c, hyx_slot, ezx_slot
   size_t off_row[8];  // den_ey, den_hy, hxy_b, hxy_c, ezy_b, ezy_c, hxy_slot, ezy_slot
   size_t off_rowinfo, off_evoff, off_events, off_src, off_det, off_comp;
+  size_t off_probe, off_cap, off_rcmap, off_rcedge, off_spd;  // RC tracker
+  int nprobe;  // distinct columns the RC lines read
   size_t n_events;
   size_t total;
 };
@@ -111,6 +113,8 @@ struct fdtd2d_plan {
   // tiles whose window holds one take the CPML-class kernel, so the interior
   // kernel carries no event code
   std::vector<int2> ev_cells;
+  int nrc = 0;          // RC cutoff lines
+  fdtd2d::RcArgs rc{};  // their device state
 };
 
 namespace fdtd2d {

@@ -59,6 +59,12 @@ struct StepArgs {
   int nsrc;
   float *det;
   int ndet;
+  // RC tracker capture: probe_col[pitch] = probe index of a column (-1: none);
+  // the stepper stores Ez of every probe column for the step it computes into
+  // rc_cap[((step-1) & 7)][probe][ny] (row = grid row), read after the launch.
+  const int *probe_col;
+  float *rc_cap;
+  int nprobe;
   float *comp;
   int ncomp;
   long long comp_plane;
@@ -109,6 +115,18 @@ constexpr int kWavesPerWG = 4;
 // Launches the step kernel of work-group class `cls` (0 interior, 1 CPML)
 // for T steps on `stream`, D rows of prefetch; src = buffer set read.
 hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream);
+
+// RC cutoff lines (Dev_RCcalc semantics) for steps n0+1 .. n0+t from the
+// capture of the launch that computed them; one work-group per line.
+struct RcArgs {
+  const float *cap;
+  const int *map;  // [2*nrc]: probe of the maximum column, probe of the scan column
+  float *edge;     // [nrc] edge levels
+  int *spd;        // [(nsteps+1)*nrc]
+  int nrc, nprobe, ny, jcut;
+  float level;
+};
+hipError_t launch_rc(const RcArgs &r, int n0, int t, hipStream_t stream);
 
 // Pool kernels: launch k of a chunk, class cls, T steps, on `grid`
 // persistent work-groups that take work-groups of tile entries (entry.w =

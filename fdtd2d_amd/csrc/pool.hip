@@ -561,6 +561,7 @@ int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
   *shot = -1;
   int rc = fdtd2d::validate_problem(p);
   if (rc) return rc;
+  if (p->nrc > 0) return set_error(FDTD2D_EUNSUPPORTED, "RC cutoff lines run on single plans only");
   if (pp->medium_kind >= 0 && p->medium_kind != pp->medium_kind)
     return set_error(FDTD2D_EUNSUPPORTED, "a pool holds shots of one medium kind");
   HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");

@@ -50,6 +50,9 @@ void Setup::bind() {
   p.ncomp = static_cast<int32_t>(comp_i.size());
   p.comp_i = comp_i.data();
   p.comp_j = comp_j.data();
+  p.nrc = static_cast<int32_t>(rc_i.size());
+  p.rc_i = rc_i.empty() ? nullptr : rc_i.data();
+  p.rc_imax = rc_imax.empty() ? nullptr : rc_imax.data();
 }
 
 // One side of one axis, F/FDTD_cuda.cu:1226-1261 (x, side 1) and its three
@@ -340,6 +343,34 @@ int setup_from_ini(const std::string &path, Setup &S) {
     }
   S.det_rows = dqI;
   S.det_cols = dqK;
+
+  // RC cutoff lines, calculateRCdetect2Darrai (F/FDTD_cuda.cu:3492-3536) on
+  // the scaled [RC_calc] keys (int /= ds truncates, :356-361; LIsize of K is
+  // 1 and its step 5, :137, 140, so one K position, K = 0). Dev_RCcalc
+  // (F/kernel.cu:968-1044) takes the maximum in column RCdetArrayI[e] and scans
+  // column RCdetArrayI[e] + e; the maximum search starts at row
+  // SourceCen.J + 150 (:2170); the edge level starts at 10.0 (:1513). Lines
+  // outside the grid are dropped (the reference would read out of bounds).
+  {
+    const int rcI = static_cast<int>(st.rcI / ds);
+    const int sizeI = static_cast<int>(st.rcPI / ds), di = static_cast<int>(st.rcDI / ds);
+    if (di > 0) {
+      const int q = sizeI / di + 1;
+      const int istart = rcI - sizeI / 2;
+      for (int e = 0; e < q; ++e) {
+        const int im = istart + di * e, is = im + e;
+        if (im < 0 || im >= Imax || is < 0 || is >= Imax) continue;
+        S.rc_imax.push_back(im);
+        S.rc_i.push_back(is);
+      }
+    }
+    p.rc_jcut = srcJ + 150;
+    p.rc_level = st.rcLevel;
+    p.rc_edge0 = 10.0f;
+    S.params["RC_Istart"] = rcI - sizeI / 2;
+    S.params["RC_di"] = di;
+    S.params["RC_qantI"] = di > 0 ? sizeI / di + 1 : 0;
+  }
 
   // buildSourceProfile, F/FDTD_cuda.cu:3539-3655 (the uninitialised tail and
   // the read at n == nMax are defined as 0 here).

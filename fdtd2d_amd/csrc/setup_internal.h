@@ -40,6 +40,7 @@ struct Setup {
   CpmlAxis cx, cy;
   std::vector<float> den_ex, den_hx, den_ey, den_hy;
   std::vector<int32_t> src_i, src_j, det_i, det_j, comp_i, comp_j;
+  std::vector<int32_t> rc_i, rc_imax;  // RC cutoff lines (scan column, maximum column)
   std::vector<float> src_table;
   std::vector<float> source_profile;  // SourceP[nMax] envelope (SourceP.txt)
   std::map<std::string, double> params;

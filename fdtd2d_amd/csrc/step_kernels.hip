@@ -104,6 +104,7 @@ struct Cols {
   float den_ex[EDGE ? C : 1], den_hx[EDGE ? C : 1];  // interior: uniform scalars
   float hyx_b[EDGE ? C : 1], hyx_c[EDGE ? C : 1], ezx_b[EDGE ? C : 1], ezx_c[EDGE ? C : 1];
   int hyx_s[EDGE ? C : 1], ezx_s[EDGE ? C : 1];
+  int probe[EDGE ? C : 1];  // RC capture probe of the column, -1 none
 };
 
 // Per-row scalars of the row a level works on (uniform across the wave):
@@ -275,6 +276,19 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
           gp(a.comp)[a.comp_plane + q] = o.hx[c];
           gp(a.comp)[2 * a.comp_plane + q] = o.hy[c];
         }
+      }
+    }
+  }
+  if constexpr (EDGE) {
+    // RC capture of probe columns (after the sources, as the reference's
+    // Dev_RCcalc reads Ez after the detectors of the step)
+    if (a.nprobe > 0 && own_row) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int col = cl.c0 + c;
+        if (cl.probe[c] >= 0 && col >= own0 && col < own1)
+          gp(a.rc_cap)[(static_cast<long long>((step - 1) & (kMaxT - 1)) * a.nprobe + cl.probe[c]) * a.ny + r] =
+              o.ez[c];
       }
     }
   }
@@ -531,6 +545,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       cl.hyx_c[c] = in ? gp(a.hyx_c)[col] : 0.f;
       cl.ezx_b[c] = in ? gp(a.ezx_b)[col] : 0.f;
       cl.ezx_c[c] = in ? gp(a.ezx_c)[col] : 0.f;
+      cl.probe[c] = in ? gp(a.probe_col)[col] : -1;
     }
   }
   Streams sm, dst;
@@ -671,6 +686,71 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, EDGE ? 4 : 5)
   }
 }
 
+// RC cutoff lines for the t steps of one launch, in step order (the edge level
+// carries from step to step). Dev_FindMax keeps a value only when a later one
+// is strictly greater, so the maximum is the first of the largest values in
+// scan order (row 0, then rows jcut .. ny-3) and a NaN at row 0 stays; the
+// scan (j from ny-3 down while |Ez| < edge) stops at the largest j >= 1 whose
+// |Ez| is not below the edge, else at 0.
+__global__ void __launch_bounds__(256) rc_kernel(const RcArgs r, int n0, int t) {
+  __shared__ float s_v[256];
+  __shared__ int s_i[256];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const int pm = r.map[2 * e], ps = r.map[2 * e + 1];
+  const int ny = r.ny;
+  const int ncand = 1 + max(0, (ny - 2) - r.jcut);
+  float edge = r.edge[e];
+  for (int s = 1; s <= t; ++s) {
+    const int step = n0 + s;
+    const long long slot = (step - 1) & (kMaxT - 1);
+    const float *cm = r.cap + (slot * r.nprobe + pm) * ny;
+    const float *cs = r.cap + (slot * r.nprobe + ps) * ny;
+    // (value, candidate index): larger value wins, equal values the earlier
+    float bv = 0.f;
+    int bi = 0x7fffffff;
+    for (int q = tid; q < ncand; q += blockDim.x) {
+      const float v = q == 0 ? cm[0] : cm[r.jcut + q - 1];
+      if (v != v) continue;  // NaN never replaces (row 0 handled below)
+      if (bi == 0x7fffffff || v > bv || (v == bv && q < bi)) {
+        bv = v;
+        bi = q;
+      }
+    }
+    s_v[tid] = bv;
+    s_i[tid] = bi;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (tid < w) {
+        const float ov = s_v[tid + w];
+        const int oi = s_i[tid + w];
+        if (oi != 0x7fffffff &&
+            (s_i[tid] == 0x7fffffff || ov > s_v[tid] || (ov == s_v[tid] && oi < s_i[tid]))) {
+          s_v[tid] = ov;
+          s_i[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    const float c0 = cm[0];
+    const float temp = c0 != c0 ? c0 : s_v[0];
+    __syncthreads();
+    const float et = temp * r.level;
+    if (et > edge) edge = et;
+    int bj = 0;
+    for (int j = 1 + tid; j <= ny - 3; j += blockDim.x)
+      if (!(fabsf(cs[j]) < edge)) bj = max(bj, j);
+    s_i[tid] = bj;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (tid < w) s_i[tid] = max(s_i[tid], s_i[tid + w]);
+      __syncthreads();
+    }
+    if (tid == 0) r.spd[static_cast<long long>(step) * r.nrc + e] = s_i[0];
+    __syncthreads();
+  }
+  if (tid == 0) r.edge[e] = edge;
+}
+
 __global__ void __launch_bounds__(64) pool_tick(PoolQueue *q, int nk, int dg) {
   int *t = &q->ticket[0][0][0];
   for (int i = threadIdx.x; i < nk * 16; i += blockDim.x) t[i] = 0;
@@ -755,6 +835,13 @@ hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int c
 hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,
                            hipStream_t stream) {
   hipLaunchKernelGGL(pool_set, dim3(1), dim3(64), 0, stream, q, items0, items1, count0, count1, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_rc(const RcArgs &r, int n0, int t, hipStream_t stream) {
+  if (r.nrc <= 0 || t <= 0) return hipSuccess;
+  if (t > kMaxT) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rc_kernel, dim3(r.nrc), dim3(256), 0, stream, r, n0, t);
   return hipGetLastError();
 }
 

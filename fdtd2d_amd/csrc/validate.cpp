@@ -53,6 +53,15 @@ int validate_problem(const fdtd2d_problem *p) {
     if (p->comp_i[c] < 0 || p->comp_i[c] > p->nx - 2 || p->comp_j[c] < 0 || p->comp_j[c] > p->ny - 2)
       return set_error(FDTD2D_EINVAL, "component detector " + std::to_string(c) +
                                           " outside [0,nx-2]x[0,ny-2]");
+  if (p->nrc < 0) return set_error(FDTD2D_EINVAL, "negative RC line count");
+  if (p->nrc > 0) {
+    if (!p->rc_i || !p->rc_imax) return set_error(FDTD2D_EINVAL, "RC lines need rc_i, rc_imax");
+    if (p->nrc > 4096) return set_error(FDTD2D_EINVAL, "at most 4096 RC lines");
+    for (int e = 0; e < p->nrc; ++e)
+      if (p->rc_i[e] < 0 || p->rc_i[e] >= p->nx || p->rc_imax[e] < 0 || p->rc_imax[e] >= p->nx)
+        return set_error(FDTD2D_EINVAL, "RC line " + std::to_string(e) + " outside the grid");
+    if (p->rc_jcut < 0) return set_error(FDTD2D_EINVAL, "rc_jcut < 0");
+  }
   return FDTD2D_OK;
 }
 

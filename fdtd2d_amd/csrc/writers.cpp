@@ -1,8 +1,10 @@
 // writers.cpp - the reference's detector output files, written by the host
 // driver from the traces a plan returns (formats of F/kernel.cu:2409-2493,
 // 2653-2676 and F/FDTD_cuda.cu:3647-3650).
+#include <cmath>
 #include <cstdio>
 #include <ctime>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -129,6 +131,82 @@ extern "C" int fdtd2d_write_outputs(const fdtd2d_setup *setup, const char *dir, 
                  static_cast<int>(get("Idetect")), static_cast<int>(get("Jdetect")), 1, 10, 10);
     std::fprintf(f, "\n\nCalculation on GPU\n Calculation time: %f seconds\n\n", seconds);
     std::fclose(f);
+  }
+  return FDTD2D_OK;
+}
+
+// RC tracker post-processing of the reference (F/kernel.cu:3180-3270, called
+// at :2628-2644 after the step loop): RClineFilterPostTime (centred moving
+// average of width FilterWindow, float sums in the reference's order),
+// CalculateRCfromPwrLines (RC_Line.txt: largest raw row, largest filtered
+// row, that times ds) and CalculateTaufromPwrLines (TAU_Line.txt: 2 i for the
+// first i >= 2000 where the filtered row stops rising, and that times dt in
+// ps), written with the iostream default formatting the reference uses; and
+// RcDet_data.txt, the line positions (F/FDTD_cuda.cu:3492-3536).
+extern "C" int fdtd2d_write_rc_outputs(const fdtd2d_setup *setup, const char *dir, const int32_t *spd) {
+  using fdtd2d::set_error;
+  if (!setup || !dir) return set_error(FDTD2D_EINVAL, "null argument");
+  const fdtd2d::Setup &S = setup->s;
+  const fdtd2d_problem &p = S.p;
+  const int nMax = p.nsteps, Q = p.nrc;
+  if (Q > 0 && !spd) return set_error(FDTD2D_EINVAL, "null RC traces");
+  auto get = [&](const char *k, double d) {
+    auto it = S.params.find(k);
+    return it == S.params.end() ? d : it->second;
+  };
+  const std::string d(dir);
+  auto path = [&](const char *name) { return d.empty() ? std::string(name) : d + "/" + name; };
+  {
+    std::ofstream out(path("RcDet_data.txt"));
+    if (!out) return set_error(FDTD2D_EIO, "cannot write RcDet_data.txt");
+    const int q = static_cast<int>(get("RC_qantI", 0)), i0 = static_cast<int>(get("RC_Istart", 0)),
+              di = static_cast<int>(get("RC_di", 0));
+    for (int i = 0; i < q; ++i) out << i0 + di * i << " ";
+    out << std::endl;
+    out << 0 << " ";  // one K position, K = 0
+  }
+  if (Q == 0) return FDTD2D_OK;
+  const int w = static_cast<int>(get("FilterWindow", 200));
+  // RClineFilterPostTime
+  std::vector<float> fl(static_cast<size_t>(nMax) * Q, 0.f);
+  const float weight = static_cast<float>(1.0 / std::abs(w));
+  for (int k = 0; k < Q; ++k) {
+    int w_beg = w / 2, w_end = -w / 2;
+    for (int i = 0; i < nMax; ++i) {
+      float acc = 0;
+      for (int j = w_beg; j > w_end; j--)
+        if (j > 0 && j < nMax) acc += static_cast<float>(spd[static_cast<size_t>(j) * Q + k]) * weight;
+      fl[static_cast<size_t>(i) * Q + k] = acc;
+      w_beg++;
+      w_end++;
+    }
+  }
+  const double ds = get("ds", 1.0);
+  {
+    std::ofstream out(path("RC_Line.txt"));
+    if (!out) return set_error(FDTD2D_EIO, "cannot write RC_Line.txt");
+    for (int k = 0; k < Q; ++k) {
+      int temp = 0;
+      float ft = 0;
+      for (int i = 0; i < nMax; ++i)
+        if (spd[static_cast<size_t>(i) * Q + k] > temp) temp = spd[static_cast<size_t>(i) * Q + k];
+      for (int i = 0; i < nMax; ++i)
+        if (fl[static_cast<size_t>(i) * Q + k] > ft) ft = fl[static_cast<size_t>(i) * Q + k];
+      out << temp << " " << ft << " " << ft * ds << std::endl;
+    }
+  }
+  {
+    std::ofstream out(path("TAU_Line.txt"));
+    if (!out) return set_error(FDTD2D_EIO, "cannot write TAU_Line.txt");
+    for (int k = 0; k < Q; ++k) {
+      float ft = 0;
+      int i = 2000;
+      do {
+        ft = static_cast<float>(i * 2);
+        i++;
+      } while (i < nMax && fl[static_cast<size_t>(i) * Q + k] - fl[static_cast<size_t>(i - 1) * Q + k] >= 0);
+      out << ft << " " << ft * p.dt * kTimePrint << std::endl;
+    }
   }
   return FDTD2D_OK;
 }

@@ -82,6 +82,11 @@ class Problem:
     det_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     comp_i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     comp_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    rc_i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    rc_imax: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    rc_jcut: int = 0
+    rc_level: float = 0.1
+    rc_edge0: float = 10.0
 
     # -------------------------------------------------------------- C struct
     def to_c(self):
@@ -127,6 +132,9 @@ class Problem:
         s.det_i, s.det_j = _ip(i32(self.det_i)), _ip(i32(self.det_j))
         s.ncomp = int(len(self.comp_i))
         s.comp_i, s.comp_j = _ip(i32(self.comp_i)), _ip(i32(self.comp_j))
+        s.nrc = int(len(self.rc_i))
+        s.rc_i, s.rc_imax = _ip(i32(self.rc_i)), _ip(i32(self.rc_imax))
+        s.rc_jcut, s.rc_level, s.rc_edge0 = self.rc_jcut, self.rc_level, self.rc_edge0
         return s, keep
 
     @classmethod
@@ -151,6 +159,8 @@ class Problem:
             src_table=_copy_f(s.src_table, (ns + 1) * s.nsrc).reshape(ns + 1, s.nsrc),
             det_i=_copy_i(s.det_i, s.ndet), det_j=_copy_i(s.det_j, s.ndet),
             comp_i=_copy_i(s.comp_i, s.ncomp), comp_j=_copy_i(s.comp_j, s.ncomp),
+            rc_i=_copy_i(s.rc_i, s.nrc), rc_imax=_copy_i(s.rc_imax, s.nrc),
+            rc_jcut=s.rc_jcut, rc_level=s.rc_level, rc_edge0=s.rc_edge0,
         )
         for name, n in (("be_x", P), ("ce_x", P), ("bh_x", P - 1), ("ch_x", P - 1),
                         ("be_y", P), ("ce_y", P), ("bh_y", P - 1), ("ch_y", P - 1)):
@@ -214,6 +224,12 @@ class Setup:
         L.check(L.lib().fdtd2d_plan_footprint(self.c_problem, C.byref(out)), "fdtd2d_plan_footprint")
         return int(out.value)
 
+    def write_rc_outputs(self, out_dir: str, spd: np.ndarray) -> None:
+        """RC_Line.txt, TAU_Line.txt and RcDet_data.txt from RC tracker traces."""
+        v = np.ascontiguousarray(spd, np.int32)
+        L.check(L.lib().fdtd2d_write_rc_outputs(self._h, str(out_dir).encode(), v.ctypes.data_as(C.POINTER(C.c_int32))),
+                "fdtd2d_write_rc_outputs")
+
     def write_outputs(self, out_dir: str, det: np.ndarray, comp: np.ndarray, seconds: float = 0.0) -> None:
         d = np.ascontiguousarray(det, np.float32)
         c = np.ascontiguousarray(comp, np.float32)
@@ -265,7 +281,7 @@ class Plan:
             ptr = C.byref(s)
             self._setup = None
         self.nx, self.ny, self.npml, self.nsteps = s.nx, s.ny, s.npml, s.nsteps
-        self.ndet, self.ncomp = s.ndet, s.ncomp
+        self.ndet, self.ncomp, self.nrc = s.ndet, s.ncomp, s.nrc
         h = C.c_void_p()
         L.check(L.lib().fdtd2d_plan_create(ptr, device, C.byref(h)), "fdtd2d_plan_create")
         self._h = h
@@ -304,6 +320,12 @@ class Plan:
         L.check(L.lib().fdtd2d_read_detectors(self._h, _fp(det), det.size, _fp(comp), comp.size),
                 "fdtd2d_read_detectors")
         return det, comp
+
+    def read_rc(self) -> np.ndarray:
+        """RC tracker traces, (nsteps+1, nrc) int32, row n = step n."""
+        out = np.zeros((self.nsteps + 1, self.nrc), np.int32)
+        L.check(L.lib().fdtd2d_read_rc(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size), "fdtd2d_read_rc")
+        return out
 
     def read_field(self, which: int) -> np.ndarray:
         out = np.zeros(FIELD_SHAPES[which](self.nx, self.ny), np.float32)

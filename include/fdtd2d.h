@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FDTD2D_ABI_VERSION 1
+#define FDTD2D_ABI_VERSION 2
 
 /* status codes */
 enum {
@@ -107,6 +107,18 @@ typedef struct fdtd2d_problem {
   /* Component detectors (F/kernel.cu:719-759): Ez, Hx, Hy at a cell. */
   int32_t ncomp;
   const int32_t *comp_i, *comp_j;
+
+  /* RC cutoff tracker (Dev_RCcalc, F/kernel.cu:700-713, 968-1044, run after
+     the detectors of every step, :2170), nrc lines, 0 = off. Line e keeps an
+     edge level, starting at rc_edge0: each step it rises to rc_level times the
+     signed maximum of Ez over row 0 and rows [rc_jcut, ny-3] of column
+     rc_imax[e] when that is larger; the step's value is then the largest row
+     j in [1, ny-3] of column rc_i[e] whose |Ez| is not below the level (0 when
+     none). Row n of the trace <- step n; row 0 = 0. */
+  int32_t nrc;
+  const int32_t *rc_i, *rc_imax;
+  int32_t rc_jcut;
+  float rc_level, rc_edge0;
 } fdtd2d_problem;
 
 typedef struct fdtd2d_plan fdtd2d_plan;
@@ -165,6 +177,12 @@ void fdtd2d_setup_destroy(fdtd2d_setup *s);
 /* Reference output files (Detect.txt, detAll.txt, detEz/Hx/Hy.txt,
  * SourceP.txt, log.txt ...; F/kernel.cu:2409-2493, F/FDTD_cuda.cu:3647-3650)
  * written into dir from traces laid out as fdtd2d_read_detectors returns them. */
+/* RC tracker products of the reference's post-processing, from traces laid out
+ * as fdtd2d_read_rc returns them: RC_Line.txt and TAU_Line.txt
+ * (CalculateRCfromPwrLines / CalculateTaufromPwrLines after the moving-average
+ * filter RClineFilterPostTime, F/kernel.cu:3180-3270, 2628-2644) and
+ * RcDet_data.txt (line positions, F/FDTD_cuda.cu:3492-3536). */
+int fdtd2d_write_rc_outputs(const fdtd2d_setup *s, const char *dir, const int32_t *spd);
 int fdtd2d_write_outputs(const fdtd2d_setup *s, const char *dir, const float *det,
                          const float *comp, double seconds);
 
@@ -213,6 +231,8 @@ int fdtd2d_write_psi(fdtd2d_plan *plan, int which, const float *in, size_t count
 int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, int band, int band_cpml,
                       const int *ev_i, const int *ev_j, int nev, int *entries, size_t capacity, size_t *count,
                       int *geometry);
+/* RC tracker traces, [(nsteps+1) * nrc] (row n = step n, row 0 = 0). */
+int fdtd2d_read_rc(fdtd2d_plan *plan, int32_t *spd, size_t count);
 /* Device bytes this plan holds (the footprint admission control counts). */
 int fdtd2d_plan_footprint(const fdtd2d_problem *p, uint64_t *bytes);
 int fdtd2d_plan_destroy(fdtd2d_plan *plan);

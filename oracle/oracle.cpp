@@ -16,6 +16,7 @@
 //   source    F/kernel.cu:621-659   Dev_AESASourceCalc (value precomputed per step)
 //   detectors F/kernel.cu:762-805   Dev_AESADetectorSim
 //   comps     F/kernel.cu:719-759   Dev_AESAComponentDetector
+//   rc lines  F/kernel.cu:700-713, 968-1044   Dev_FindMax, Dev_RCcalc
 // and the step order of the host loop F/kernel.cu:2065-2157.
 //
 // What pins each function independently of this file (tests/test_oracle_kat.py
@@ -32,6 +33,11 @@
 //                              coefficient closed forms (tests/test_setup.py)
 //   ez_main PEC id             PEC material forces Ez = 0
 //   source, detectors, comps   marked-grid brute force of every index rule
+//   rc lines                   step-by-step brute force of Dev_RCcalc (a
+//                              planted column mistake fails it); the full
+//                              33568-step run of the reference's own
+//                              Settings.ini reproduces its RC_Line.txt and
+//                              TAU_Line.txt line for line
 //   whole step                 FP32 within tolerance of FP64; threads bitwise
 // No function here is left without such a check. Left unpinned, by nature:
 // the reference's sm_20 FMA contraction, CUDA 7.5 device cos/pow/exp in its
@@ -75,6 +81,7 @@ struct SimBase {
   virtual int read_psi(int which, double *out, size_t count) const = 0;
   virtual int write_psi(int which, const double *in, size_t count) = 0;
   virtual int read_det(double *det, size_t dc, double *comp, size_t cc) const = 0;
+  virtual int read_rc(int32_t *spd, size_t count) const = 0;
   int nthreads = 1;
   int n = 0;
 };
@@ -122,6 +129,9 @@ struct Sim : SimBase {
   std::vector<R> src_table;
   std::vector<int32_t> src_i, src_j, det_i, det_j, comp_i, comp_j;
   std::vector<R> det, comp;
+  std::vector<int32_t> rc_i, rc_imax, spd;
+  std::vector<R> rc_edge;
+  R rc_level;
   R da, db, ca0, cb0, cj, jz_decay, dxr, dyr;
   double jz_dt;
 
@@ -168,6 +178,13 @@ struct Sim : SimBase {
     det_j.assign(q.det_j, q.det_j + q.ndet);
     comp_i.assign(q.comp_i, q.comp_i + q.ncomp);
     comp_j.assign(q.comp_j, q.comp_j + q.ncomp);
+    if (q.nrc > 0) {
+      rc_i.assign(q.rc_i, q.rc_i + q.nrc);
+      rc_imax.assign(q.rc_imax, q.rc_imax + q.nrc);
+    }
+    rc_edge.assign(q.nrc, R(q.rc_edge0));
+    rc_level = R(q.rc_level);
+    spd.assign((size_t)(q.nsteps + 1) * q.nrc, 0);
     det.assign((size_t)(q.nsteps + 1) * q.ndet, R(0));
     comp.assign((size_t)3 * (q.nsteps + 1) * q.ncomp, R(0));
     da = R(1);
@@ -320,6 +337,22 @@ struct Sim : SimBase {
       comp[plane + row] = Hx[comp_j[c] * NX + comp_i[c]];
       comp[2 * plane + row] = Hy[comp_j[c] * NH + comp_i[c]];
     }
+    // ---- RC cutoff lines, after the detectors (F/kernel.cu:2170): Dev_RCcalc
+    // (:968-1044) with Dev_FindMax (:700-713). The maximum is signed and seeded
+    // from row 0 of the maximum column; the scan starts below row ny-2.
+    for (int e = 0; e < p.nrc; ++e) {
+      const size_t im = (size_t)rc_imax[e], is = (size_t)rc_i[e];
+      R temp = Ez[im];
+      for (int j = p.rc_jcut; j < ny - 2; ++j)
+        if (Ez[j * NX + im] > temp) temp = Ez[j * NX + im];
+      const R edge_tmp = temp * rc_level;
+      if (edge_tmp > rc_edge[e]) rc_edge[e] = edge_tmp;
+      int j = ny - 2;
+      do {
+        j--;
+      } while (j > 0 && std::abs(Ez[j * NX + is]) < rc_edge[e]);
+      spd[(size_t)n_step * p.nrc + e] = j;
+    }
     n = n_step;
   }
 
@@ -378,6 +411,11 @@ struct Sim : SimBase {
       for (auto &v : f[s]) v = R(in[k++]);
     return FDTD2D_OK;
   }
+  int read_rc(int32_t *out, size_t count) const override {
+    if (count != spd.size()) return fail(FDTD2D_EINVAL, "oracle_read_rc: count must be (nsteps+1)*nrc");
+    for (size_t k = 0; k < count; ++k) out[k] = spd[k];
+    return FDTD2D_OK;
+  }
   int read_det(double *d, size_t dc, double *c, size_t cc) const override {
     if (d) {
       if (dc != det.size()) return fail(FDTD2D_EINVAL, "oracle_read_detectors: det count");
@@ -427,6 +465,11 @@ int validate(const fdtd2d_problem *p) {
     if (p->comp_i[c] < 0 || p->comp_i[c] > p->nx - 2 || p->comp_j[c] < 0 ||
         p->comp_j[c] > p->ny - 2)
       return fail(FDTD2D_EINVAL, "component detector outside [0,nx-2]x[0,ny-2]");
+  if (p->nrc < 0 || (p->nrc > 0 && (!p->rc_i || !p->rc_imax))) return fail(FDTD2D_EINVAL, "bad RC lines");
+  for (int e = 0; e < p->nrc; ++e)
+    if (p->rc_i[e] < 0 || p->rc_i[e] >= p->nx || p->rc_imax[e] < 0 || p->rc_imax[e] >= p->nx)
+      return fail(FDTD2D_EINVAL, "RC line outside the grid");
+  if (p->nrc > 0 && p->rc_jcut < 0) return fail(FDTD2D_EINVAL, "rc_jcut < 0");
   return FDTD2D_OK;
 }
 
@@ -477,6 +520,9 @@ int oracle_write_psi(oracle_sim *s, int which, const double *in, size_t count) {
 }
 int oracle_read_detectors(const oracle_sim *s, double *det, size_t dc, double *comp, size_t cc) {
   return s ? s->impl->read_det(det, dc, comp, cc) : FDTD2D_EINVAL;
+}
+int oracle_read_rc(const oracle_sim *s, int32_t *spd, size_t count) {
+  return s ? s->impl->read_rc(spd, count) : FDTD2D_EINVAL;
 }
 void oracle_destroy(oracle_sim *s) {
   if (s) {

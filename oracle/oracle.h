@@ -37,6 +37,8 @@ int oracle_write_psi(oracle_sim *s, int which, const double *in, size_t count);
 /* det [(nsteps+1)*ndet], comp [3][(nsteps+1)*ncomp] (Ez, Hx, Hy). */
 int oracle_read_detectors(const oracle_sim *s, double *det, size_t det_count, double *comp,
                           size_t comp_count);
+/* RC tracker traces [(nsteps+1)*nrc], row n = step n. */
+int oracle_read_rc(const oracle_sim *s, int32_t *spd, size_t count);
 void oracle_destroy(oracle_sim *s);
 const char *oracle_last_error(void);
 

@@ -35,6 +35,7 @@ def lib() -> C.CDLL:
             ("oracle_read_psi", C.c_int, [C.c_void_p, C.c_int, _d, C.c_size_t]),
             ("oracle_write_psi", C.c_int, [C.c_void_p, C.c_int, _d, C.c_size_t]),
             ("oracle_read_detectors", C.c_int, [C.c_void_p, _d, C.c_size_t, _d, C.c_size_t]),
+            ("oracle_read_rc", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_size_t]),
             ("oracle_destroy", None, [C.c_void_p]),
             ("oracle_last_error", C.c_char_p, []),
         ]
@@ -68,7 +69,7 @@ class Oracle:
             ptr = problem.c_problem
             s = ptr.contents
         self.nx, self.ny, self.npml, self.nsteps = s.nx, s.ny, s.npml, s.nsteps
-        self.ndet, self.ncomp = s.ndet, s.ncomp
+        self.ndet, self.ncomp, self.nrc = s.ndet, s.ncomp, s.nrc
         self.precision = precision
         h = C.c_void_p()
         _check(lib().oracle_create(ptr, precision, C.byref(h)), "oracle_create")
@@ -107,6 +108,12 @@ class Oracle:
         _check(lib().oracle_read_detectors(self._h, _dp(det), det.size, _dp(comp), comp.size),
                "oracle_read_detectors")
         return det, comp
+
+    def read_rc(self) -> np.ndarray:
+        """RC tracker traces, (nsteps+1, nrc) int32, row n = step n."""
+        out = np.zeros((self.nsteps + 1, self.nrc), np.int32)
+        _check(lib().oracle_read_rc(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size), "oracle_read_rc")
+        return out
 
     def close(self) -> None:
         if getattr(self, "_h", None):

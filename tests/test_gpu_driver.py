@@ -51,3 +51,39 @@ def test_shipped_settings_on_gpu_match_oracle(gpu_device, tmp_path, name):
     s.write_outputs(str(r), rdet.astype(np.float32), rcomp.astype(np.float32))
     for f in FILES:
         assert (g / f).read_bytes() == (r / f).read_bytes(), f
+
+
+def test_shipped_settings_rc_lines_match_oracle(gpu_device, tmp_path):
+    s = F.Setup.from_ini(GOLDEN / "settings_release.ini")
+    with F.Plan(s, gpu_device) as plan:
+        plan.run(STEPS)
+        plan.sync()
+        spd = plan.read_rc()
+    ref = Oracle(s, 32, threads=8)
+    ref.run(STEPS)
+    want = ref.read_rc()
+    ref.close()
+    assert np.array_equal(spd, want)
+
+
+def test_full_length_run_reproduces_reference_rc_and_tau_lines(gpu_device, tmp_path):
+    """All 33568 steps of the reference's own Settings.ini (FDTD_2D_Full), then
+    the RC tracker's post-processing: RC_Line.txt and TAU_Line.txt equal the
+    files the reference holds, line for line (the end-to-end fixture: INI
+    parsing, medium builders, source table, stepping, RC tracking, filter and
+    formatting)."""
+    import json
+
+    fx = json.loads((GOLDEN / "reference_lines.json").read_text())
+    s = F.Setup.from_ini(GOLDEN / "settings_full.ini")
+    p = s.c_problem.contents
+    assert p.nsteps == 33568
+    with F.Plan(s, gpu_device) as plan:
+        plan.run(p.nsteps)
+        plan.sync()
+        spd = plan.read_rc()
+    s.write_rc_outputs(str(tmp_path), spd)
+    rc = (tmp_path / "RC_Line.txt").read_text().splitlines()
+    tau = (tmp_path / "TAU_Line.txt").read_text().splitlines()
+    assert rc == fx["rc_line_text"]
+    assert tau == fx["tau_line_text"]

@@ -189,3 +189,31 @@ def test_nonuniform_den_inside(gpu_device):
     with F.Plan(q, gpu_device) as plan:
         plan.set_option("tblock", 6)
         assert plan.info("tiles_cpml") < info["tiles_cpml"]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_rc_lines_match_oracle(gpu_device, seed):
+    """RC cutoff lines: probe columns captured by the stepper every step and
+    reduced after each launch, bitwise against the oracle's per-step rule."""
+    rng = np.random.default_rng(6000 + seed)
+    p = random_problem(rng, nx=int(rng.integers(200, 420)), ny=int(rng.integers(150, 400)),
+                       npml=int(rng.choice([0, 8, 11])), nsteps=int(rng.integers(30, 70)), table=bool(seed % 2),
+                       nsrc=3, ndet=2, ncomp=1)
+    p.src_table = p.src_table * 50
+    nrc = int(rng.integers(1, 12))
+    p.rc_imax = np.sort(rng.choice(np.arange(2, p.nx - 20), nrc, replace=False)).astype(np.int32)
+    p.rc_i = (p.rc_imax + np.arange(nrc)).astype(np.int32)
+    p.rc_jcut, p.rc_level, p.rc_edge0 = int(rng.integers(0, p.ny)), 0.07, 1e-3
+    ref = Oracle(p, 32, threads=4)
+    ref.run(p.nsteps)
+    want = ref.read_rc()
+    for tblock in (8, 3):
+        with F.Plan(p, gpu_device, tblock=tblock) as plan:
+            plan.run(p.nsteps)
+            plan.sync()
+            got = plan.read_rc()
+            det, _ = plan.read_detectors()
+        print(f"seed {seed} T={tblock}: {p.nx}x{p.ny} nrc={nrc} jcut={p.rc_jcut} rows {np.unique(want).size}",
+              flush=True)
+        assert np.array_equal(got, want)
+        assert np.array_equal(det, ref.read_detectors()[0].astype(np.float32))

@@ -247,3 +247,90 @@ def test_synthetic_shots_respect_the_yee_drude_stability_bound():
         bound = (c * p.dt) ** 2 * (1 / p.dx ** 2 + 1 / p.dy ** 2) + wp2 * p.dt ** 2 / 4
         assert bound <= 1.0, (w, shot, bound)
         s.close()
+
+
+def test_rc_lines_follow_reference_positions(release):
+    """calculateRCdetect2Darrai on the scaled [RC_calc] keys: Iline 125 mm,
+    LIsize 100 mm, LIdelta 5 mm at ds 0.5 mm -> 21 lines from column 150 in
+    steps of 10; the edge scan column adds the line index; the maximum search
+    starts 150 rows past the source row; Rlevel 0.07; edge level from 10."""
+    p = release.c_problem.contents
+    assert p.nrc == 21
+    imax = [p.rc_imax[e] for e in range(21)]
+    isc = [p.rc_i[e] for e in range(21)]
+    assert imax == [150 + 10 * e for e in range(21)]
+    assert isc == [150 + 11 * e for e in range(21)]
+    assert p.rc_jcut == int(release.get("Jsource")) + 150 == 250
+    assert abs(p.rc_level - 0.07) < 1e-7 and p.rc_edge0 == 10.0
+
+
+def _rc_post(spd, window, ds, dt):
+    """RClineFilterPostTime, CalculateRCfromPwrLines, CalculateTaufromPwrLines
+    (F/kernel.cu:3180-3270) restated with float32 sums."""
+    nmax, q = spd.shape[0] - 1, spd.shape[1]
+    weight = np.float32(1.0 / abs(window))
+    fl = np.zeros((nmax, q), np.float32)
+    for k in range(q):
+        wb, we = window // 2, -(window // 2)
+        for i in range(nmax):
+            acc = np.float32(0)
+            for j in range(wb, we, -1):
+                if 0 < j < nmax:
+                    acc = np.float32(acc + np.float32(spd[j, k]) * weight)
+            fl[i, k] = acc
+            wb += 1
+            we += 1
+    rc, tau = [], []
+    for k in range(q):
+        temp = max(0, int(spd[:nmax, k].max()))
+        ft = max(np.float32(0), fl[:, k].max())
+        rc.append((temp, float(ft), float(ft) * ds))
+        i = 2000
+        while True:
+            f = float(np.float32(i * 2))
+            i += 1
+            if not (i < nmax and fl[i, k] - fl[i - 1, k] >= 0):
+                break
+        tau.append((f, f * dt * 1e12))
+    return rc, tau
+
+
+def test_rc_outputs_known_answers(release, tmp_path):
+    """A constant line gives its own value raw and filtered, and the filtered
+    line stops rising where the moving window starts to leave the run:
+    TAU = 2 (nMax - window/2 - 1)."""
+    p = release.c_problem.contents
+    nmax, q = p.nsteps, p.nrc
+    spd = np.zeros((nmax + 1, q), np.int32)
+    spd[1:, :] = 100 + np.arange(q, dtype=np.int32)[None, :]
+    release.write_rc_outputs(str(tmp_path), spd)
+    rc = [l.split() for l in (tmp_path / "RC_Line.txt").read_text().splitlines()]
+    tau = [l.split() for l in (tmp_path / "TAU_Line.txt").read_text().splitlines()]
+    assert len(rc) == len(tau) == q
+    w = int(release.get("FilterWindow"))
+    for k in range(q):
+        assert int(rc[k][0]) == 100 + k
+        assert abs(float(rc[k][1]) - (100 + k)) < 1e-3 * (100 + k)
+        assert float(tau[k][0]) == 2 * (nmax - w // 2 - 1)
+    pos = (tmp_path / "RcDet_data.txt").read_text().split("\n")
+    assert [int(x) for x in pos[0].split()] == [150 + 10 * e for e in range(21)]
+
+
+def test_rc_outputs_match_restatement(release, tmp_path):
+    """A moving, noisy line through the writers against the restatement above
+    (formatting as std::ostream prints float and double)."""
+    p = release.c_problem.contents
+    nmax, q = p.nsteps, p.nrc
+    rng = np.random.default_rng(5)
+    t = np.arange(nmax + 1)[:, None]
+    spd = (2000 + 0.05 * t + 20 * np.sin(t / 700.0 + np.arange(q)[None, :]) + rng.integers(-3, 4, (nmax + 1, q)))
+    spd = spd.astype(np.int32)
+    spd[0] = 0
+    release.write_rc_outputs(str(tmp_path), spd)
+    rc_want, tau_want = _rc_post(spd[:, 4:5], int(release.get("FilterWindow")), release.get("ds"), p.dt)
+    rc = [l.split() for l in (tmp_path / "RC_Line.txt").read_text().splitlines()][4:5]
+    tau = [l.split() for l in (tmp_path / "TAU_Line.txt").read_text().splitlines()][4:5]
+    for k in range(1):
+        assert int(rc[k][0]) == rc_want[k][0]
+        assert rc[k][1] == f"{rc_want[k][1]:.6g}" and rc[k][2] == f"{rc_want[k][2]:.6g}"
+        assert tau[k][0] == f"{tau_want[k][0]:.6g}" and tau[k][1] == f"{tau_want[k][1]:.6g}"
