@@ -76,6 +76,49 @@ class ProblemC(C.Structure):
         ("rc_jcut", C.c_int32),
         ("rc_level", C.c_float),
         ("rc_edge0", C.c_float),
+        ("line_size", C.c_int32),
+        ("line_i0", C.c_int32),
+        ("line_j", C.c_int32),
+        ("line_begin", C.c_int32),
+        ("line_end", C.c_int32),
+    ]
+
+
+class ProcessingC(C.Structure):
+    """Mirror of ``struct fdtd2d_processing``."""
+
+    _fields_ = [
+        ("perform", C.c_int32),
+        ("sec_der", C.c_int32),
+        ("sdf_type", C.c_int32),
+        ("sdf_win", C.c_int32),
+        ("sdf_runs", C.c_int32),
+        ("cf", C.c_int32),
+        ("cf_tau", C.c_int32),
+        ("cf_k", C.c_float),
+        ("arc", C.c_int32),
+        ("arc_tau", C.c_int32),
+        ("arc_k", C.c_float),
+        ("tau_threshold", C.c_float),
+        ("filter_type", C.c_int32),
+        ("filter_win", C.c_int32),
+        ("filter_runs", C.c_int32),
+        ("save_original", C.c_int32),
+        ("save_filtered", C.c_int32),
+        ("dim_sieve", C.c_int32),
+        ("time_sieve", C.c_int32),
+        ("sim_detect", C.c_int32),
+        ("save_detector", C.c_int32),
+        ("load_weights", C.c_int32),
+        ("detector_size", C.c_int32),
+        ("begin_after", C.c_int32),
+        ("finish_at", C.c_int32),
+        ("pos_i", C.c_int32),
+        ("pos_j", C.c_int32),
+        ("pos_k", C.c_int32),
+        ("size_i", C.c_int32),
+        ("times", C.c_int32),
+        ("dt", C.c_double),
     ]
 
 
@@ -106,6 +149,10 @@ SIGNATURES = [
     ("fdtd2d_write_outputs", C.c_int, [C.c_void_p, C.c_char_p, _f, _f, C.c_double]),
     ("fdtd2d_write_rc_outputs", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32)]),
     ("fdtd2d_read_rc", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_size_t]),
+    ("fdtd2d_read_line", C.c_int, [C.c_void_p, _f, C.c_size_t]),
+    ("fdtd2d_setup_processing", C.POINTER(ProcessingC), [C.c_void_p]),
+    ("fdtd2d_processing_tof", C.c_int, [C.POINTER(ProcessingC), _f, C.c_int, _f, C.c_int, _f, _f, _f, _f, _f]),
+    ("fdtd2d_write_processing_outputs", C.c_int, [C.POINTER(ProcessingC), C.c_char_p, _f, C.c_int]),
     ("fdtd2d_plan_create", C.c_int, [C.POINTER(ProblemC), C.c_int, C.POINTER(C.c_void_p)]),
     ("fdtd2d_plan_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
     ("fdtd2d_plan_set_tblock", C.c_int, [C.c_void_p, C.c_int]),
@@ -137,6 +184,7 @@ SIGNATURES = [
                                          C.POINTER(C.c_int)]),
     ("fdtd2d_pool_read_detectors", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t, _f, C.c_size_t]),
     ("fdtd2d_pool_read_field", C.c_int, [C.c_void_p, C.c_int, C.c_int, _f, C.c_size_t]),
+    ("fdtd2d_pool_read_line", C.c_int, [C.c_void_p, C.c_int, _f, C.c_size_t]),
     ("fdtd2d_pool_remove", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_pool_destroy", C.c_int, [C.c_void_p]),
 ]

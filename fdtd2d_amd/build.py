@@ -79,7 +79,8 @@ def build_oracle() -> Path:
     out.mkdir(parents=True, exist_ok=True)
     so = out / "liboracle.so"
     tmp = out / "liboracle.so.tmp"
-    _run(["g++", *CXXFLAGS, "-shared", str(ORACLE / "oracle.cpp"), "-o", str(tmp), "-pthread"])
+    _run(["g++", *CXXFLAGS, "-shared", str(ORACLE / "oracle.cpp"), str(ORACLE / "processing.cpp"), "-o", str(tmp),
+          "-pthread"])
     os.replace(tmp, so)
     return so
 

@@ -1,9 +1,11 @@
 // main.cpp - command-line host driver, the role of FDTD_cuda.cu's main() /
 // initlimits() (F/FDTD_cuda.cu:20-23, 384-441): read Settings.ini from the
 // working directory (or the path given), build the problem, run it on the
-// GPU named by [hardware] GPU_device, write the reference's output files.
+// GPU named by [hardware] GPU_device, write the reference's output files
+// (detectors, RC tracker, processing line).
 //
 //   fdtd2d [Settings.ini] [--out DIR] [--steps N] [--device D] [--tblock T]
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -60,6 +62,21 @@ int main(int argc, char **argv) {
     std::vector<int32_t> spd(static_cast<size_t>(p->nsteps + 1) * p->nrc);
     if (p->nrc > 0 && (rc = fdtd2d_read_rc(plan, spd.data(), spd.size()))) return die(rc, "read_rc");
     if ((rc = fdtd2d_write_rc_outputs(s, out.c_str(), spd.data()))) return die(rc, "write_rc_outputs");
+  }
+  // processing_core on the captured line (F/kernel.cu:2313-2315), over the
+  // steps of its window this run reached
+  const fdtd2d_processing *a = fdtd2d_setup_processing(s);
+  if (a->perform) {
+    const int times = p->line_size > 0 ? std::max(0, std::min(p->line_end, n) - p->line_begin) : 0;
+    const size_t rows = p->line_size > 0 ? static_cast<size_t>(std::max(0, std::min(p->line_end, p->nsteps) -
+                                                                        p->line_begin))
+                                          : 0;
+    std::vector<float> line(rows * static_cast<size_t>(p->line_size > 0 ? p->line_size : 0));
+    if ((rc = fdtd2d_read_line(plan, line.data(), line.size()))) return die(rc, "read_line");
+    std::printf("Processing...\n");
+    if ((rc = fdtd2d_write_processing_outputs(a, out.c_str(), line.data(), times)))
+      return die(rc, "processing");
+    std::printf("Processing done.\n");
   }
   std::printf("%d steps of %dx%d in %.3f s\n", n, p->nx, p->ny, secs);
   fdtd2d_plan_destroy(plan);

@@ -106,6 +106,9 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
   L.off_rcmap = take(static_cast<size_t>(2) * p->nrc * 4);
   L.off_rcedge = take(static_cast<size_t>(p->nrc) * 4);
   L.off_spd = take(static_cast<size_t>(p->nsteps + 1) * p->nrc * 4);
+  L.line_rows =
+      p->line_size > 0 ? static_cast<size_t>(std::max(0, std::min(p->line_end, p->nsteps) - p->line_begin)) : 0;
+  L.off_line = take(L.line_rows * std::max(p->line_size, 0) * 4);
   L.total = o;
   return L;
 }
@@ -579,8 +582,10 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       pl->scol_pref[i + 1] = pl->scol_pref[i] + sp;
     }
     pl->srow_pref.assign(ny + 1, 0);
+    // so is the row of the processing line
     for (int j = 0; j < ny; ++j) {
-      const bool sp = j >= ny - 1 || !same(p->den_hy[j], u_hy) || !same(p->den_ey[j], u_ey);
+      const bool sp = j >= ny - 1 || !same(p->den_hy[j], u_hy) || !same(p->den_ey[j], u_ey) ||
+                      (L.line_rows > 0 && j == p->line_j);
       pl->srow_pref[j + 1] = pl->srow_pref[j] + sp;
     }
   }
@@ -706,6 +711,14 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   pl->rc.ny = ny;
   pl->rc.jcut = p->rc_jcut;
   pl->rc.level = p->rc_level;
+  pl->line_rows = L.line_rows;
+  pl->line_size = L.line_rows > 0 ? p->line_size : 0;
+  a.line = reinterpret_cast<float *>(pl->base + L.off_line);
+  a.line_j = L.line_rows > 0 ? p->line_j : -1;
+  a.line_i0 = p->line_i0;
+  a.line_i1 = p->line_i0 + pl->line_size;
+  a.line_begin = p->line_begin;
+  a.line_end = std::min(p->line_end, p->nsteps);
   if ((rc = set_tiling(pl))) return fail(rc);
   *out = pl;
   return FDTD2D_OK;
@@ -862,6 +875,17 @@ int fdtd2d_read_rc(fdtd2d_plan *pl, int32_t *spd, size_t count) {
   HIP_TRY(hipSetDevice(pl->device), "hipSetDevice");
   HIP_TRY(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
   if (n) HIP_TRY(hipMemcpy(spd, pl->rc.spd, n * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H rc");
+  return FDTD2D_OK;
+}
+
+int fdtd2d_read_line(fdtd2d_plan *pl, float *line, size_t count) {
+  if (!pl) return set_error(FDTD2D_EINVAL, "null plan");
+  const size_t n = pl->line_rows * pl->line_size;
+  if (n && !line) return set_error(FDTD2D_EINVAL, "null line buffer");
+  if (count != n) return set_error(FDTD2D_EINVAL, "count must be rows*line_size of the processing line");
+  HIP_TRY(hipSetDevice(pl->device), "hipSetDevice");
+  HIP_TRY(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
+  if (n) HIP_TRY(hipMemcpy(line, pl->a.line, n * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H line");
   return FDTD2D_OK;
 }
 

@@ -39,6 +39,8 @@ struct Layout {
   size_t off_row[8];  // den_ey, den_hy, hxy_b, hxy_c, ezy_b, ezy_c, hxy_slot, ezy_slot
   size_t off_rowinfo, off_evoff, off_events, off_src, off_det, off_comp;
   size_t off_probe, off_cap, off_rcmap, off_rcedge, off_spd;  // RC tracker
+  size_t off_line;  // processing line captures
+  size_t line_rows;
   int nprobe;  // distinct columns the RC lines read
   size_t n_events;
   size_t total;
@@ -113,6 +115,8 @@ struct fdtd2d_plan {
   // tiles whose window holds one take the CPML-class kernel, so the interior
   // kernel carries no event code
   std::vector<int2> ev_cells;
+  size_t line_rows = 0;  // processing line: captured steps
+  int line_size = 0;      // and cells per step
   int nrc = 0;          // RC cutoff lines
   fdtd2d::RcArgs rc{};  // their device state
 };

@@ -65,6 +65,12 @@ struct StepArgs {
   const int *probe_col;
   float *rc_cap;
   int nprobe;
+  // Processing line: Ez of row line_j, columns [line_i0, line_i1), after the
+  // sources of step n for line_begin < n <= line_end, into
+  // line[(n - line_begin - 1) * (line_i1 - line_i0) + column - line_i0].
+  // The row is special, so only the CPML-class kernel carries this code.
+  float *line;
+  int line_j, line_i0, line_i1, line_begin, line_end;
   float *comp;
   int ncomp;
   long long comp_plane;

@@ -671,6 +671,14 @@ int fdtd2d_pool_read_field(fdtd2d_pool *pp, int id, int which, float *out, size_
   return fdtd2d_read_field(sh->pl, which, out, count);
 }
 
+int fdtd2d_pool_read_line(fdtd2d_pool *pp, int id, float *line, size_t count) {
+  Shot *sh = get_shot(pp, id);
+  if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");
+  int rc = wait_shot(pp, *sh);
+  if (rc) return rc;
+  return fdtd2d_read_line(sh->pl, line, count);
+}
+
 int fdtd2d_pool_remove(fdtd2d_pool *pp, int id) {
   Shot *sh = get_shot(pp, id);
   if (!sh) return set_error(FDTD2D_EINVAL, "no such shot");

@@ -184,6 +184,7 @@ struct RefSettings {
   int rcI = 300, rcPI = 40, rcDI = 5;
   float rcLevel = 0.1f;
   int rcWindow = 200;
+  fdtd2d_processing proc{};  // the reference's Processing global starts zeroed
 };
 
 void read_settings(const IniFile &r, RefSettings &s) {
@@ -231,6 +232,38 @@ void read_settings(const IniFile &r, RefSettings &s) {
   s.rcDI = r.get_int("RC_calc", "LIdelta", 5);
   s.rcLevel = static_cast<float>(r.get_real("RC_calc", "Rlevel", 0.1));
   s.rcWindow = r.get_int("RC_calc", "FilterWindow", 200);
+  // [Processing], F/FDTD_cuda.cu:144-172 (SDF* default to the prefilter's keys)
+  fdtd2d_processing &a = s.proc;
+  a.perform = r.get_bool("Processing", "Perform", false);
+  a.sec_der = r.get_bool("Processing", "SecDer", false);
+  a.cf = r.get_bool("Processing", "CF", false);
+  a.arc = r.get_bool("Processing", "ARC", false);
+  a.save_filtered = r.get_bool("Processing", "SaveFiltered", false);
+  a.save_original = r.get_bool("Processing", "SaveOriginal", false);
+  a.sim_detect = r.get_bool("Processing", "DetSim", false);
+  a.save_detector = r.get_bool("Processing", "DetSave", false);
+  a.load_weights = r.get_bool("Processing", "DetWLoad", false);
+  auto gi = [&](const char *k, long d) { return static_cast<int32_t>(r.get_int("Processing", k, d)); };
+  a.detector_size = gi("DetSize", 30);
+  a.filter_type = gi("FilterType", 1);
+  a.dim_sieve = gi("DimSieve", 1);
+  a.time_sieve = gi("TimeSieve", 1);
+  a.begin_after = gi("BeginAfter", 0);
+  a.finish_at = gi("FinishAt", 0);
+  a.pos_i = gi("PosI", 0);
+  a.pos_j = gi("PosJ", 0);
+  a.pos_k = gi("PosK", 0);
+  a.size_i = gi("SizeI", 0);
+  a.filter_win = gi("FilterWin", 300);
+  a.filter_runs = gi("FilterRuns", 1);
+  a.sdf_win = gi("SDFWin", a.filter_win);
+  a.sdf_type = gi("SDFType", a.filter_type);
+  a.sdf_runs = gi("SDFRuns", a.filter_runs);
+  a.cf_tau = gi("CFTau", 300);
+  a.arc_tau = gi("ARCTau", 300);
+  a.tau_threshold = static_cast<float>(r.get_real("Processing", "TauThreshold", 0));
+  a.arc_k = static_cast<float>(r.get_real("Processing", "ARCK", -0.5));
+  a.cf_k = static_cast<float>(r.get_real("Processing", "CFK", -0.5));
 }
 
 // Element positions of calculate2Darrai / calculateDetect2Darrai
@@ -370,6 +403,35 @@ int setup_from_ini(const std::string &path, Setup &S) {
     S.params["RC_Istart"] = rcI - sizeI / 2;
     S.params["RC_di"] = di;
     S.params["RC_qantI"] = di > 0 ? sizeI / di + 1 : 0;
+  }
+
+  // [Processing] in cells and steps (F/FDTD_cuda.cu:364-377: int /= ds and
+  // int *= 1e-9/dt truncate), and the line the stepper captures for it
+  // (F/FDTD_cuda.cu:471-483, F/kernel.cu:2138-2145). A line outside the grid,
+  // or off the single K plane, is not captured: the reference would read
+  // outside its field or store nothing.
+  {
+    fdtd2d_processing a = st.proc;
+    a.pos_i = static_cast<int32_t>(a.pos_i / ds);
+    a.pos_j = static_cast<int32_t>(a.pos_j / ds);
+    a.pos_k = static_cast<int32_t>(a.pos_k / ds);
+    a.size_i = static_cast<int32_t>(a.size_i / ds);
+    if (st.Exp_time_key == 1) {
+      a.begin_after = static_cast<int32_t>(a.begin_after * (1e-9 / dt));
+      a.finish_at = static_cast<int32_t>(a.finish_at * (1e-9 / dt));
+    }
+    a.times = a.perform ? std::max(0, std::min(a.finish_at, nMax) - a.begin_after) : 0;
+    a.dt = dt;
+    S.proc = a;
+    const bool inside = a.size_i > 0 && a.pos_i >= 0 && static_cast<long long>(a.pos_i) + a.size_i <= Imax &&
+                        a.pos_j >= 0 && a.pos_j < Jmax && a.pos_k == 0 && a.begin_after >= 0;
+    if (a.perform && a.times > 0 && inside) {
+      p.line_size = a.size_i;
+      p.line_i0 = a.pos_i;
+      p.line_j = a.pos_j;
+      p.line_begin = a.begin_after;
+      p.line_end = a.finish_at;
+    }
   }
 
   // buildSourceProfile, F/FDTD_cuda.cu:3539-3655 (the uninitialised tail and
@@ -903,6 +965,8 @@ int fdtd2d_setup_box(int nx, int ny, int npml, int nsteps, double ds_m, fdtd2d_s
   *out = s;
   return FDTD2D_OK;
 }
+
+const fdtd2d_processing *fdtd2d_setup_processing(const fdtd2d_setup *s) { return s ? &s->s.proc : nullptr; }
 
 const fdtd2d_problem *fdtd2d_setup_problem(const fdtd2d_setup *s) { return s ? &s->s.p : nullptr; }
 fdtd2d_problem *fdtd2d_setup_problem_mut(fdtd2d_setup *s) { return s ? &s->s.p : nullptr; }

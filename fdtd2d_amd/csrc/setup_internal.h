@@ -41,6 +41,7 @@ struct Setup {
   std::vector<float> den_ex, den_hx, den_ey, den_hy;
   std::vector<int32_t> src_i, src_j, det_i, det_j, comp_i, comp_j;
   std::vector<int32_t> rc_i, rc_imax;  // RC cutoff lines (scan column, maximum column)
+  fdtd2d_processing proc{};            // [Processing], scaled to cells and steps
   std::vector<float> src_table;
   std::vector<float> source_profile;  // SourceP[nMax] envelope (SourceP.txt)
   std::map<std::string, double> params;

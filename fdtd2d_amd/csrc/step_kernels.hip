@@ -291,6 +291,18 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
               o.ez[c];
       }
     }
+    // processing line (Dev_SaveLine runs after the sources, before the
+    // detectors; all three read the same Ez)
+    if (r == a.line_j && own_row && step > a.line_begin && step <= a.line_end) {
+      const long long t = step - a.line_begin - 1;
+      const int w = a.line_i1 - a.line_i0;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int col = cl.c0 + c;
+        if (col >= a.line_i0 && col < a.line_i1 && col >= own0 && col < own1)
+          gp(a.line)[t * w + (col - a.line_i0)] = o.ez[c];
+      }
+    }
   }
 }
 

@@ -62,6 +62,13 @@ int validate_problem(const fdtd2d_problem *p) {
         return set_error(FDTD2D_EINVAL, "RC line " + std::to_string(e) + " outside the grid");
     if (p->rc_jcut < 0) return set_error(FDTD2D_EINVAL, "rc_jcut < 0");
   }
+  if (p->line_size < 0) return set_error(FDTD2D_EINVAL, "negative processing line size");
+  if (p->line_size > 0) {
+    if (p->line_i0 < 0 || static_cast<long long>(p->line_i0) + p->line_size > p->nx || p->line_j < 0 ||
+        p->line_j >= p->ny)
+      return set_error(FDTD2D_EINVAL, "processing line outside the grid");
+    if (p->line_begin < 0) return set_error(FDTD2D_EINVAL, "processing line_begin < 0");
+  }
   return FDTD2D_OK;
 }
 

@@ -87,6 +87,13 @@ class Problem:
     rc_jcut: int = 0
     rc_level: float = 0.1
     rc_edge0: float = 10.0
+    # processing line: Ez of row line_j, columns [line_i0, line_i0 + line_size)
+    # after steps line_begin < n <= line_end (line_size 0 = off)
+    line_size: int = 0
+    line_i0: int = 0
+    line_j: int = 0
+    line_begin: int = 0
+    line_end: int = 0
 
     # -------------------------------------------------------------- C struct
     def to_c(self):
@@ -135,6 +142,8 @@ class Problem:
         s.nrc = int(len(self.rc_i))
         s.rc_i, s.rc_imax = _ip(i32(self.rc_i)), _ip(i32(self.rc_imax))
         s.rc_jcut, s.rc_level, s.rc_edge0 = self.rc_jcut, self.rc_level, self.rc_edge0
+        s.line_size, s.line_i0, s.line_j = self.line_size, self.line_i0, self.line_j
+        s.line_begin, s.line_end = self.line_begin, self.line_end
         return s, keep
 
     @classmethod
@@ -161,6 +170,8 @@ class Problem:
             comp_i=_copy_i(s.comp_i, s.ncomp), comp_j=_copy_i(s.comp_j, s.ncomp),
             rc_i=_copy_i(s.rc_i, s.nrc), rc_imax=_copy_i(s.rc_imax, s.nrc),
             rc_jcut=s.rc_jcut, rc_level=s.rc_level, rc_edge0=s.rc_edge0,
+            line_size=s.line_size, line_i0=s.line_i0, line_j=s.line_j,
+            line_begin=s.line_begin, line_end=s.line_end,
         )
         for name, n in (("be_x", P), ("ce_x", P), ("bh_x", P - 1), ("ch_x", P - 1),
                         ("be_y", P), ("ce_y", P), ("bh_y", P - 1), ("ch_y", P - 1)):
@@ -176,6 +187,14 @@ class Problem:
 
     def cell_updates(self, steps: Optional[int] = None) -> int:
         return self.nx * self.ny * (self.nsteps if steps is None else steps)
+
+
+def line_shape(s) -> tuple:
+    """(rows, line_size) of the processing-line captures of a problem."""
+    if s.line_size <= 0:
+        return (0, 0)
+    rows = max(0, min(s.line_end, s.nsteps) - s.line_begin)
+    return (rows, s.line_size if rows else 0)
 
 
 class Setup:
@@ -223,6 +242,10 @@ class Setup:
         out = C.c_uint64()
         L.check(L.lib().fdtd2d_plan_footprint(self.c_problem, C.byref(out)), "fdtd2d_plan_footprint")
         return int(out.value)
+
+    def processing(self) -> L.ProcessingC:
+        """A copy of the setup's [Processing] parameters (cells and steps)."""
+        return L.ProcessingC.from_buffer_copy(L.lib().fdtd2d_setup_processing(self._h).contents)
 
     def write_rc_outputs(self, out_dir: str, spd: np.ndarray) -> None:
         """RC_Line.txt, TAU_Line.txt and RcDet_data.txt from RC tracker traces."""
@@ -282,6 +305,7 @@ class Plan:
             self._setup = None
         self.nx, self.ny, self.npml, self.nsteps = s.nx, s.ny, s.npml, s.nsteps
         self.ndet, self.ncomp, self.nrc = s.ndet, s.ncomp, s.nrc
+        self.line_shape = line_shape(s)
         h = C.c_void_p()
         L.check(L.lib().fdtd2d_plan_create(ptr, device, C.byref(h)), "fdtd2d_plan_create")
         self._h = h
@@ -325,6 +349,12 @@ class Plan:
         """RC tracker traces, (nsteps+1, nrc) int32, row n = step n."""
         out = np.zeros((self.nsteps + 1, self.nrc), np.int32)
         L.check(L.lib().fdtd2d_read_rc(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size), "fdtd2d_read_rc")
+        return out
+
+    def read_line(self) -> np.ndarray:
+        """Processing line captures, (rows, line_size) float32; row t = step line_begin + t + 1."""
+        out = np.zeros(self.line_shape, np.float32)
+        L.check(L.lib().fdtd2d_read_line(self._h, _fp(out), out.size), "fdtd2d_read_line")
         return out
 
     def read_field(self, which: int) -> np.ndarray:
@@ -408,7 +438,7 @@ class Pool:
             ptr = C.byref(s)
         sid = C.c_int()
         L.check(L.lib().fdtd2d_pool_add(self._h, ptr, C.byref(sid)), "fdtd2d_pool_add")
-        self._shots[sid.value] = (s.nx, s.ny, s.nsteps, s.ndet, s.ncomp)
+        self._shots[sid.value] = (s.nx, s.ny, s.nsteps, s.ndet, s.ncomp, line_shape(s))
         return sid.value
 
     def step(self, nlaunch: int) -> None:
@@ -429,7 +459,7 @@ class Pool:
         return d.value, t.value, bool(c.value)
 
     def read_detectors(self, shot: int):
-        nx, ny, nsteps, ndet, ncomp = self._shots[shot]
+        nx, ny, nsteps, ndet, ncomp = self._shots[shot][:5]
         det = np.zeros((nsteps + 1, ndet), np.float32)
         comp = np.zeros((3, nsteps + 1, ncomp), np.float32)
         L.check(L.lib().fdtd2d_pool_read_detectors(self._h, shot, _fp(det), det.size, _fp(comp), comp.size),
@@ -440,6 +470,11 @@ class Pool:
         nx, ny = self._shots[shot][:2]
         out = np.zeros(FIELD_SHAPES[which](nx, ny), np.float32)
         L.check(L.lib().fdtd2d_pool_read_field(self._h, shot, which, _fp(out), out.size), "fdtd2d_pool_read_field")
+        return out
+
+    def read_line(self, shot: int) -> np.ndarray:
+        out = np.zeros(self._shots[shot][5], np.float32)
+        L.check(L.lib().fdtd2d_pool_read_line(self._h, shot, _fp(out), out.size), "fdtd2d_pool_read_line")
         return out
 
     def remove(self, shot: int) -> None:

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FDTD2D_ABI_VERSION 2
+#define FDTD2D_ABI_VERSION 3
 
 /* status codes */
 enum {
@@ -119,7 +119,38 @@ typedef struct fdtd2d_problem {
   const int32_t *rc_i, *rc_imax;
   int32_t rc_jcut;
   float rc_level, rc_edge0;
+
+  /* Processing line (Dev_SaveLine, F/kernel.cu:666-688, copied to the host
+     each step n with line_begin < n <= line_end, :2138-2145): Ez of row
+     line_j, columns [line_i0, line_i0 + line_size), read after the sources of
+     step n. Capture row t = n - line_begin - 1 for the
+     min(line_end, nsteps) - line_begin captured steps; line_size 0 = off.
+     (The reference kernel stores only when line_j == 1, its thread index
+     never reaching another row; this is the line its arguments name, and
+     equals the reference's capture when line_j == 1 and line_i0 == 1.) */
+  int32_t line_size, line_i0, line_j, line_begin, line_end;
 } fdtd2d_problem;
+
+/* [Processing] of Settings.ini (struct Analysis, F/processing.cuh:8-51, read
+ * at F/FDTD_cuda.cu:144-172, scaled at :364-377): time-of-flight search over
+ * the captured line (processing_core, F/processing.cu:6-144). Positions and
+ * size are in cells (ints divided by ds, truncated), begin_after / finish_at
+ * in steps; times = captured steps; dt = the step (s) of the TOF columns. */
+typedef struct fdtd2d_processing {
+  int32_t perform;
+  int32_t sec_der, sdf_type, sdf_win, sdf_runs;  /* second-derivative method and its post-filter */
+  int32_t cf, cf_tau;                             /* constant fraction: delay (steps), attenuation */
+  float cf_k;
+  int32_t arc, arc_tau;
+  float arc_k;
+  float tau_threshold;                            /* fraction of the maximum a sample must exceed */
+  int32_t filter_type, filter_win, filter_runs;   /* 1 moving average, 2 median, other: none */
+  int32_t save_original, save_filtered, dim_sieve, time_sieve;
+  int32_t sim_detect, save_detector, load_weights, detector_size;
+  int32_t begin_after, finish_at, pos_i, pos_j, pos_k, size_i;
+  int32_t times;
+  double dt;
+} fdtd2d_processing;
 
 typedef struct fdtd2d_plan fdtd2d_plan;
 typedef struct fdtd2d_setup fdtd2d_setup;
@@ -183,6 +214,30 @@ void fdtd2d_setup_destroy(fdtd2d_setup *s);
  * filter RClineFilterPostTime, F/kernel.cu:3180-3270, 2628-2644) and
  * RcDet_data.txt (line positions, F/FDTD_cuda.cu:3492-3536). */
 int fdtd2d_write_rc_outputs(const fdtd2d_setup *s, const char *dir, const int32_t *spd);
+/* The setup's [Processing] parameters (never NULL for a valid setup; perform
+ * = 0 when the INI has none). */
+const fdtd2d_processing *fdtd2d_setup_processing(const fdtd2d_setup *s);
+/* Time-of-flight search of processing_core (F/processing.cu:6-144) over a
+ * captured line laid out as fdtd2d_read_line returns it ([times][size_i]).
+ * pass 0: the line itself, n = size_i coordinates; pass 1: the simulated
+ * detector (a weighted sum over detector_size neighbours, :64-100), n =
+ * size_i - detector_size; weights (detector_size floats) may be NULL for the
+ * uniform 1/detector_size. sd, cf, arc (n floats each, may be NULL) get
+ * begin_after + the step index each method finds (the last index when none);
+ * original and filtered ([n][times], may be NULL) the per-coordinate |Ez|
+ * (or detector) series before and after the prefilter. Inputs on which the
+ * reference reads outside its arrays (series shorter than a filter window or
+ * a delay + 1, zero sieves or windows, detector_size >= size_i) are
+ * FDTD2D_EINVAL here. */
+int fdtd2d_processing_tof(const fdtd2d_processing *a, const float *line, int times, const float *weights,
+                          int pass, float *sd, float *cf, float *arc, float *original, float *filtered);
+/* processing_core's files, written into dir from a captured line: SD.txt,
+ * CF.txt, ARC.txt, original_field.txt, filtered_field.txt and, with
+ * sim_detect, det_SD/det_CF/det_ARC.txt and detector_original_field.txt /
+ * detector_filtered_field.txt (F/processing.cu:44-140, 262-283); with
+ * load_weights the detector weights come from dir/DetWeights.txt as the
+ * reference reads its working directory's. */
+int fdtd2d_write_processing_outputs(const fdtd2d_processing *a, const char *dir, const float *line, int times);
 int fdtd2d_write_outputs(const fdtd2d_setup *s, const char *dir, const float *det,
                          const float *comp, double seconds);
 
@@ -233,6 +288,11 @@ int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, i
                       int *geometry);
 /* RC tracker traces, [(nsteps+1) * nrc] (row n = step n, row 0 = 0). */
 int fdtd2d_read_rc(fdtd2d_plan *plan, int32_t *spd, size_t count);
+/* Processing line captures, [rows * line_size] with rows =
+ * max(0, min(line_end, nsteps) - line_begin); row t = step line_begin + t + 1
+ * (the reference's proc_field, F/FDTD_cuda.cu:471-483). Rows of steps not
+ * yet run hold 0. */
+int fdtd2d_read_line(fdtd2d_plan *plan, float *line, size_t count);
 /* Device bytes this plan holds (the footprint admission control counts). */
 int fdtd2d_plan_footprint(const fdtd2d_problem *p, uint64_t *bytes);
 int fdtd2d_plan_destroy(fdtd2d_plan *plan);
@@ -284,6 +344,7 @@ int fdtd2d_pool_shot_state(fdtd2d_pool *pool, int shot, int *steps_done, int *st
 int fdtd2d_pool_read_detectors(fdtd2d_pool *pool, int shot, float *det, size_t det_count, float *comp,
                                size_t comp_count);
 int fdtd2d_pool_read_field(fdtd2d_pool *pool, int shot, int which, float *out, size_t count);
+int fdtd2d_pool_read_line(fdtd2d_pool *pool, int shot, float *line, size_t count);
 int fdtd2d_pool_remove(fdtd2d_pool *pool, int shot);
 int fdtd2d_pool_destroy(fdtd2d_pool *pool);
 

@@ -57,6 +57,7 @@
 //    F/kernel.cu:155 bit for bit.
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -82,6 +83,7 @@ struct SimBase {
   virtual int write_psi(int which, const double *in, size_t count) = 0;
   virtual int read_det(double *det, size_t dc, double *comp, size_t cc) const = 0;
   virtual int read_rc(int32_t *spd, size_t count) const = 0;
+  virtual int read_line(double *out, size_t count) const = 0;
   int nthreads = 1;
   int n = 0;
 };
@@ -129,6 +131,8 @@ struct Sim : SimBase {
   std::vector<R> src_table;
   std::vector<int32_t> src_i, src_j, det_i, det_j, comp_i, comp_j;
   std::vector<R> det, comp;
+  std::vector<R> line;  // processing line captures [rows][line_size]
+  int line_rows = 0;
   std::vector<int32_t> rc_i, rc_imax, spd;
   std::vector<R> rc_edge;
   R rc_level;
@@ -185,6 +189,8 @@ struct Sim : SimBase {
     rc_edge.assign(q.nrc, R(q.rc_edge0));
     rc_level = R(q.rc_level);
     spd.assign((size_t)(q.nsteps + 1) * q.nrc, 0);
+    line_rows = q.line_size > 0 ? std::max(0, std::min(q.line_end, q.nsteps) - q.line_begin) : 0;
+    line.assign((size_t)line_rows * (line_rows ? q.line_size : 0), R(0));
     det.assign((size_t)(q.nsteps + 1) * q.ndet, R(0));
     comp.assign((size_t)3 * (q.nsteps + 1) * q.ncomp, R(0));
     da = R(1);
@@ -327,6 +333,13 @@ struct Sim : SimBase {
       size_t c = src_j[e] * NX + src_i[e];
       Ez[c] = Ez[c] - src_table[(size_t)n_step * p.nsrc + e];
     }
+    // ---- processing line (Dev_SaveLine F/kernel.cu:666-688, copied out at
+    // :2138-2145 for begin_after < n <= finish_at): Ez of row line_j, columns
+    // [line_i0, line_i0 + line_size), the line the kernel's arguments name
+    if (line_rows > 0 && n_step > p.line_begin && n_step - p.line_begin <= line_rows) {
+      const size_t t = (size_t)(n_step - p.line_begin - 1);
+      for (int k = 0; k < p.line_size; ++k) line[t * p.line_size + k] = Ez[p.line_j * NX + p.line_i0 + k];
+    }
     // ---- detectors (F/kernel.cu:762-805) and components (F/kernel.cu:719-759)
     for (int d = 0; d < p.ndet; ++d)
       det[(size_t)n_step * p.ndet + d] = Ez[det_j[d] * NX + det_i[d]];
@@ -411,6 +424,11 @@ struct Sim : SimBase {
       for (auto &v : f[s]) v = R(in[k++]);
     return FDTD2D_OK;
   }
+  int read_line(double *out, size_t count) const override {
+    if (count != line.size()) return fail(FDTD2D_EINVAL, "oracle_read_line: count must be rows*line_size");
+    for (size_t k = 0; k < count; ++k) out[k] = double(line[k]);
+    return FDTD2D_OK;
+  }
   int read_rc(int32_t *out, size_t count) const override {
     if (count != spd.size()) return fail(FDTD2D_EINVAL, "oracle_read_rc: count must be (nsteps+1)*nrc");
     for (size_t k = 0; k < count; ++k) out[k] = spd[k];
@@ -470,6 +488,10 @@ int validate(const fdtd2d_problem *p) {
     if (p->rc_i[e] < 0 || p->rc_i[e] >= p->nx || p->rc_imax[e] < 0 || p->rc_imax[e] >= p->nx)
       return fail(FDTD2D_EINVAL, "RC line outside the grid");
   if (p->nrc > 0 && p->rc_jcut < 0) return fail(FDTD2D_EINVAL, "rc_jcut < 0");
+  if (p->line_size < 0) return fail(FDTD2D_EINVAL, "negative processing line size");
+  if (p->line_size > 0 && (p->line_i0 < 0 || (long long)p->line_i0 + p->line_size > p->nx || p->line_j < 0 ||
+                           p->line_j >= p->ny || p->line_begin < 0))
+    return fail(FDTD2D_EINVAL, "processing line outside the grid");
   return FDTD2D_OK;
 }
 
@@ -520,6 +542,9 @@ int oracle_write_psi(oracle_sim *s, int which, const double *in, size_t count) {
 }
 int oracle_read_detectors(const oracle_sim *s, double *det, size_t dc, double *comp, size_t cc) {
   return s ? s->impl->read_det(det, dc, comp, cc) : FDTD2D_EINVAL;
+}
+int oracle_read_line(const oracle_sim *s, double *line, size_t count) {
+  return s ? s->impl->read_line(line, count) : FDTD2D_EINVAL;
 }
 int oracle_read_rc(const oracle_sim *s, int32_t *spd, size_t count) {
   return s ? s->impl->read_rc(spd, count) : FDTD2D_EINVAL;

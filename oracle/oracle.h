@@ -39,6 +39,12 @@ int oracle_read_detectors(const oracle_sim *s, double *det, size_t det_count, do
                           size_t comp_count);
 /* RC tracker traces [(nsteps+1)*nrc], row n = step n. */
 int oracle_read_rc(const oracle_sim *s, int32_t *spd, size_t count);
+/* Processing line captures [rows][line_size], row t = step line_begin+t+1. */
+int oracle_read_line(const oracle_sim *s, double *line, size_t count);
+/* processing_core's time-of-flight search (oracle/processing.cpp), with the
+ * arguments and results of fdtd2d_processing_tof. */
+int oracle_processing_tof(const fdtd2d_processing *a, const float *line, int times, const float *weights,
+                          int pass, float *sd, float *cf, float *arc, float *original, float *filtered);
 void oracle_destroy(oracle_sim *s);
 const char *oracle_last_error(void);
 

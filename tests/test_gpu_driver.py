@@ -87,3 +87,49 @@ def test_full_length_run_reproduces_reference_rc_and_tau_lines(gpu_device, tmp_p
     tau = (tmp_path / "TAU_Line.txt").read_text().splitlines()
     assert rc == fx["rc_line_text"]
     assert tau == fx["tau_line_text"]
+
+
+PROC_FILES = ["SD.txt", "CF.txt", "ARC.txt", "original_field.txt", "filtered_field.txt", "det_SD.txt", "det_CF.txt",
+              "det_ARC.txt", "detector_original_field.txt", "detector_filtered_field.txt"]
+
+
+def test_cli_processing_line_matches_oracle(gpu_device, tmp_path):
+    """The fdtd2d CLI on the reference's Settings.ini with the processing
+    window moved to the first nanosecond (and windows and delays shortened to
+    fit the 600 steps run): the TOF and field files it writes equal those the
+    same post-processing writes from the oracle's captured line."""
+    import subprocess
+
+    from fdtd2d_amd import processing as P
+
+    edits = {"BeginAfter": "0", "FinishAt": "1", "FilterWin": "50", "SDFWin": "30", "CFTau": "100",
+             "ARCTau": "80", "DetWLoad": "0"}
+    out, lines, in_proc = [], (GOLDEN / "settings_full.ini").read_text().splitlines(), False
+    for ln in lines:
+        st = ln.strip()
+        if st.startswith("["):
+            in_proc = st.lower() == "[processing]"
+        key = st.split("=")[0].strip() if "=" in st else None
+        out.append(f"{key} = {edits[key]}" if in_proc and key in edits else ln)
+    ini = tmp_path / "Settings.ini"
+    ini.write_text("\n".join(out) + "\n")
+    g, r = tmp_path / "gpu", tmp_path / "oracle"
+    g.mkdir()
+    r.mkdir()
+    exe = Path(F.LIB_PATH).parent / "fdtd2d"
+    print(f"CLI {STEPS} steps with the processing window at (0, 2098]", flush=True)
+    res = subprocess.run([str(exe), str(ini), "--out", str(g), "--steps", str(STEPS), "--device", str(gpu_device)],
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    s = F.Setup.from_ini(str(ini))
+    a = s.processing()
+    assert (a.begin_after, a.finish_at) == (0, int(1e-9 / s.get("dt")))
+    ref = Oracle(s, 32, threads=8)
+    ref.run(STEPS)
+    line = ref.read_line()[:STEPS].astype(np.float32)
+    ref.close()
+    P.write_outputs(a, str(r), line)
+    for f in PROC_FILES:
+        assert (g / f).read_bytes() == (r / f).read_bytes(), f
+    sd = np.loadtxt(g / "CF.txt")
+    assert sd.shape == (210, 3) and sd[0, 0] == 40

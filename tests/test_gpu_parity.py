@@ -217,3 +217,43 @@ def test_rc_lines_match_oracle(gpu_device, seed):
               flush=True)
         assert np.array_equal(got, want)
         assert np.array_equal(det, ref.read_detectors()[0].astype(np.float32))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_processing_line_matches_oracle(gpu_device, seed):
+    """Processing line captured by the CPML-class kernel every step of its
+    window: bitwise the oracle's captures, for lines anywhere (CPML rows, the
+    outer ring, the full width) and windows not aligned to the launches."""
+    rng = np.random.default_rng(7000 + seed)
+    cols = int(rng.choice([1, 2, 4]))
+    nx = int(rng.integers(260, 640)) if cols < 4 else int(rng.integers(760, 900))
+    p = random_problem(rng, nx=nx, ny=int(rng.integers(150, 420)), npml=int(rng.choice([0, 8, 11])),
+                       nsteps=int(rng.integers(20, 60)), table=bool(seed % 2))
+    p.src_table = p.src_table * 50
+    size = int(rng.integers(1, p.nx + 1))
+    p.line_size, p.line_i0 = size, int(rng.integers(0, p.nx - size + 1))
+    # the outer ring rows, or the row of a source with the line through it
+    p.line_j = int([0, p.ny - 1, p.src_j[0], p.src_j[0]][seed % 4])
+    if p.line_j == p.src_j[0]:
+        p.line_i0 = int(rng.integers(max(0, p.src_i[0] - size + 1), min(p.src_i[0], p.nx - size) + 1))
+    p.line_begin = int(rng.integers(0, p.nsteps - 1))
+    p.line_end = int(rng.integers(p.line_begin + 1, p.nsteps + 20))
+    opts = {"cols_per_lane": cols, "tblock": int(rng.integers(1, 9 if cols < 4 else 5)),
+            "band": int(rng.choice([0, 16, 64])), "band_cpml": int(rng.choice([0, 8, 32]))}
+    ref = Oracle(p, 32, threads=4)
+    ref.run(p.nsteps)
+    want = ref.read_line()
+    with F.Plan(p, gpu_device) as plan:
+        for k, v in opts.items():
+            plan.set_option(k, v)
+        print(f"seed {seed}: {p.nx}x{p.ny} P={p.npml} line row {p.line_j} cols [{p.line_i0}, "
+              f"{p.line_i0 + size}) steps ({p.line_begin}, {p.line_end}] {opts}", flush=True)
+        plan.run(p.nsteps)
+        plan.sync()
+        got = plan.read_line()
+        ez = plan.read_field(F.EZ)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want.astype(np.float32))
+    assert np.array_equal(ez, ref.read_field(F.EZ).astype(np.float32))
+    if p.line_j == p.src_j[0]:
+        assert np.abs(want).max() > 0

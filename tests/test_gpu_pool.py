@@ -130,3 +130,25 @@ def test_pool_refuses_mixed_medium_kinds(gpu_device):
         with pytest.raises(F.FDTD2DError) as ei:
             pool.add(b)
         assert ei.value.code == 6
+
+
+def test_pool_processing_lines(gpu_device):
+    """Shots with processing lines in one pool: each shot's captures equal
+    the oracle's, whatever runs beside it."""
+    probs = _problems(515, 5, table=False, steps=(30, 70))
+    rng = np.random.default_rng(516)
+    for p in probs[:4]:
+        p.src_table = p.src_table * 50
+        size = int(rng.integers(1, p.nx + 1))
+        p.line_size, p.line_i0, p.line_j = size, int(rng.integers(0, p.nx - size + 1)), int(rng.integers(0, p.ny))
+        p.line_begin = int(rng.integers(0, p.nsteps // 2))
+        p.line_end = int(rng.integers(p.line_begin + 1, p.nsteps + 5))
+    with F.Pool(gpu_device, tblock=8, graph_chunk=4) as pool:
+        ids = [pool.add(p) for p in probs]
+        _drain(pool)
+        for sid, p in zip(ids, probs):
+            ref = Oracle(p, 32, threads=4)
+            ref.run(p.nsteps)
+            got = pool.read_line(sid)
+            assert np.array_equal(got, ref.read_line().astype(np.float32)), f"shot {sid}"
+            ref.close()
