@@ -8,4 +8,4 @@ from ._lib import (  # noqa: F401
     EZ, HX, HY, JZ, MEDIUM_DENSITY, MEDIUM_TABLE, PSI_EZX, PSI_EZY, PSI_HXY, PSI_HYX,
     FDTD2DError, LIB_PATH, lib,
 )
-from .problem import Plan, Pool, Problem, Setup  # noqa: F401
+from .problem import Plan, Pool, Problem, Setup, Snapshots  # noqa: F401

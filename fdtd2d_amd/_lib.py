@@ -137,6 +137,14 @@ SIGNATURES = [
     ("fdtd2d_abi_version", C.c_int, []),
     ("fdtd2d_device_count", C.c_int, [C.POINTER(C.c_int)]),
     ("fdtd2d_setup_from_ini", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_setup_from_ini_objects", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_setup_set", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("fdtd2d_write_s2d_outputs", C.c_int, [C.c_void_p, C.c_char_p, _f, _f]),
+    ("fdtd2d_write_s2d_difference", C.c_int, [C.c_void_p, C.c_char_p, _f, _f]),
+    ("fdtd2d_snapshot_steps", C.c_int, [C.c_void_p, C.c_int, _i, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("fdtd2d_snapshots_open", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_snapshots_add", C.c_int, [C.c_void_p, _f, C.c_size_t]),
+    ("fdtd2d_snapshots_close", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("fdtd2d_setup_ini_status", C.c_int, [C.c_void_p]),
     ("fdtd2d_ini_lookup", C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_double,
                                     C.POINTER(C.c_double), C.POINTER(C.c_int)]),

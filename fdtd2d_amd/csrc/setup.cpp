@@ -14,6 +14,7 @@
 
 #include "errors.h"
 #include "ini.h"
+#include "geometry.h"
 #include "medium.h"
 #include "setup_internal.h"
 
@@ -291,7 +292,7 @@ void aesa_array(int PI, int PK, int DI, int DK, int cenI, int cenK, std::vector<
 
 }  // namespace
 
-int setup_from_ini(const std::string &path, Setup &S) {
+int setup_from_ini(const std::string &path, Setup &S, bool objects) {
   IniFile r(path);
   S.ini_status = r.status();
   RefSettings st;
@@ -343,13 +344,18 @@ int setup_from_ini(const std::string &path, Setup &S) {
   cp.sigma_max_y = 0.75 * (0.8 * (cp.m + 1) / (dy * eta));
   build_cpml(cp, dt, dx, dy, Imax, Jmax, S);
 
-  // buildObject, F/FDTD_cuda.cu:2602-2659: the plasma medium into ID3.
+  // buildObject, F/FDTD_cuda.cu:2602-2659, in its order: with objects, the
+  // chamber figure (Draw2D == 1) and then the plasma medium over it; the plate
+  // wall (DrawWall == 1) last. initlimits (:384-441) builds without objects
+  // for the first pass of the source-to-detector regime and never sets the
+  // wall flag in that regime.
   S.material.assign(static_cast<size_t>(Imax) * Jmax, 0);
-  int mrc = build_reference_medium(st.med, Imax, Jmax, S.ne, S.material);
-  if (mrc) return mrc;
-  if (st.draw2d != 0 || st.drawWall != 0)
-    return set_error(FDTD2D_EUNSUPPORTED,
-                     "Settings.ini: Draw2D / DrawWall metal geometry is not implemented");
+  if (objects) {
+    if (st.draw2d == 1) draw_chamber_figure(st.preset, st.patL, srcI, Imax, Jmax, S.material);
+    int mrc = build_reference_medium(st.med, Imax, Jmax, S.ne, S.material);
+    if (mrc) return mrc;
+  }
+  if (st.S2D != 1 && st.drawWall == 1) build_plate_wall(st.wallPos, Imax, Jmax, S.material);
 
   // AESA source and detector arrays with the element offset +e and row +1.
   std::vector<int> aI, aK, dI, dK;
@@ -505,6 +511,12 @@ int setup_from_ini(const std::string &path, Setup &S) {
   P["To"] = st.toT;
   P["TimeSieve"] = st.timeSieve;
   P["GridSieve"] = st.fieldSieve;
+  P["Draw2D"] = st.draw2d;
+  P["2Dpreset"] = st.preset;
+  P["PatrubokL"] = st.patL;
+  P["DrawWall"] = st.drawWall;
+  P["WallPos"] = st.wallPos;
+  P["objects"] = objects ? 1 : 0;
   P["Iline"] = static_cast<int>(st.rcI / ds);
   P["LIsize"] = static_cast<int>(st.rcPI / ds);
   P["LIdelta"] = static_cast<int>(st.rcDI / ds);
@@ -906,10 +918,14 @@ using fdtd2d::set_error;
 extern "C" {
 
 int fdtd2d_setup_from_ini(const char *ini_path, fdtd2d_setup **out) {
+  return fdtd2d_setup_from_ini_objects(ini_path, 1, out);
+}
+
+int fdtd2d_setup_from_ini_objects(const char *ini_path, int objects, fdtd2d_setup **out) {
   if (!ini_path || !out) return set_error(FDTD2D_EINVAL, "null argument");
   auto *s = new (std::nothrow) fdtd2d_setup();
   if (!s) return set_error(FDTD2D_ENOMEM, "out of host memory");
-  int rc = fdtd2d::setup_from_ini(ini_path, s->s);
+  int rc = fdtd2d::setup_from_ini(ini_path, s->s, objects != 0);
   if (rc) {
     delete s;
     return rc;
@@ -967,6 +983,13 @@ int fdtd2d_setup_box(int nx, int ny, int npml, int nsteps, double ds_m, fdtd2d_s
 }
 
 const fdtd2d_processing *fdtd2d_setup_processing(const fdtd2d_setup *s) { return s ? &s->s.proc : nullptr; }
+
+int fdtd2d_setup_set(fdtd2d_setup *s, const char *name, double value) {
+  if (!s || !name) return set_error(FDTD2D_EINVAL, "null argument");
+  if (std::string(name) != "SlicesSaved") return set_error(FDTD2D_EINVAL, std::string("not settable: ") + name);
+  s->s.params[name] = value;
+  return FDTD2D_OK;
+}
 
 const fdtd2d_problem *fdtd2d_setup_problem(const fdtd2d_setup *s) { return s ? &s->s.p : nullptr; }
 fdtd2d_problem *fdtd2d_setup_problem_mut(fdtd2d_setup *s) { return s ? &s->s.p : nullptr; }

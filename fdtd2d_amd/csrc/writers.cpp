@@ -21,6 +21,64 @@ FILE *open_in(const std::string &dir, const char *name, const char *mode) {
   return std::fopen(path.c_str(), mode);
 }
 
+// detAll<suffix>.txt (t [us], Ex, Ey (0), Ez, Hx, Hy sums, Hz (0), all x 1000)
+// and detEz/detHx/detHy<suffix>.txt (one value per element per line, a blank
+// line per step). The main pass (F/kernel.cu:2440-2493) ends detAll lines with
+// "\n " and gives detP.txt the line breaks; the source-to-detector pass
+// (:1838-1888) ends them with "\n" and leaves detP_S2D.txt empty.
+int write_components(const fdtd2d::Setup &S, const std::string &d, const char *suffix, const float *comp,
+                     bool main_pass) {
+  using fdtd2d::set_error;
+  const fdtd2d_problem &p = S.p;
+  const int nMax = p.nsteps, ncomp = p.ncomp;
+  const double dt = p.dt;
+  const size_t plane = static_cast<size_t>(nMax + 1) * ncomp;
+  const std::string sfx(suffix);
+  FILE *f = open_in(d, ("detAll" + sfx + ".txt").c_str(), "w");
+  if (!f) return set_error(FDTD2D_EIO, "cannot write detAll" + sfx + ".txt in " + d);
+  for (int i = 0; i < nMax; ++i) {
+    double s3 = 0, s4 = 0, s5 = 0, z = 0;
+    for (int k = 0; k < ncomp; ++k) {
+      size_t q = static_cast<size_t>(i) * ncomp + k;
+      s3 += comp[q];
+      s4 += comp[plane + q];
+      s5 += comp[2 * plane + q];
+    }
+    std::fprintf(f, "%lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\n%s", (dt * i) * 1000000, z * kDetMult,
+                 z * kDetMult, s3 * kDetMult, s4 * kDetMult, s5 * kDetMult, z * kDetMult, main_pass ? " " : "");
+  }
+  std::fclose(f);
+  const std::string names[4] = {"detEz" + sfx + ".txt", "detHx" + sfx + ".txt", "detHy" + sfx + ".txt",
+                                "detP" + sfx + ".txt"};
+  FILE *fs[4];
+  for (int q = 0; q < 4; ++q) {
+    fs[q] = open_in(d, names[q].c_str(), "w");
+    if (!fs[q]) return set_error(FDTD2D_EIO, "cannot write " + names[q]);
+  }
+  const int nq = main_pass ? 4 : 3;
+  const int rows = ncomp / (S.det_cols > 0 ? S.det_cols : 1);
+  const int cols = S.det_cols > 0 ? S.det_cols : 1;
+  for (int i = 0; i < nMax; ++i) {
+    for (int j = 0; j < rows; ++j) {
+      for (int k = 0; k < cols; ++k)
+        for (int q = 0; q < 3; ++q)
+          std::fprintf(fs[q], "%12lf\t", comp[q * plane + static_cast<size_t>(i) * ncomp + j * cols + k] * kDetMult);
+      for (int q = 0; q < nq; ++q) std::fprintf(fs[q], "\n");
+    }
+    for (int q = 0; q < nq; ++q) std::fprintf(fs[q], "\n");
+  }
+  for (int q = 0; q < 4; ++q) std::fclose(fs[q]);
+  return FDTD2D_OK;
+}
+
+// Files the reference opens and never writes in this build's configuration.
+int touch(const std::string &d, const char *name) {
+  FILE *f = open_in(d, name, "w");
+  if (!f) return fdtd2d::set_error(FDTD2D_EIO, std::string("cannot write ") + name + " in " + d);
+  std::fclose(f);
+  return FDTD2D_OK;
+}
+
 }  // namespace
 
 extern "C" int fdtd2d_write_outputs(const fdtd2d_setup *setup, const char *dir, const float *det,
@@ -47,44 +105,7 @@ extern "C" int fdtd2d_write_outputs(const fdtd2d_setup *setup, const char *dir, 
     std::fclose(f);
   }
   if (comp) {
-    const size_t plane = static_cast<size_t>(nMax + 1) * ncomp;
-    // detAll.txt: t [us], Ex, Ey (0), Ez, Hx, Hy sums, Hz (0), all x 1000
-    FILE *f = open_in(d, "detAll.txt", "w");
-    if (!f) return set_error(FDTD2D_EIO, "cannot write detAll.txt in " + d);
-    for (int i = 0; i < nMax; ++i) {
-      double s3 = 0, s4 = 0, s5 = 0, z = 0;
-      for (int k = 0; k < ncomp; ++k) {
-        size_t q = static_cast<size_t>(i) * ncomp + k;
-        s3 += comp[q];
-        s4 += comp[plane + q];
-        s5 += comp[2 * plane + q];
-      }
-      std::fprintf(f, "%lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\n ", (dt * i) * 1000000, z * kDetMult,
-                   z * kDetMult, s3 * kDetMult, s4 * kDetMult, s5 * kDetMult, z * kDetMult);
-    }
-    std::fclose(f);
-    // detEz/detHx/detHy.txt: one value per element per line, blank line per step;
-    // detP.txt receives only the line breaks.
-    const char *names[3] = {"detEz.txt", "detHx.txt", "detHy.txt"};
-    FILE *fs[4];
-    for (int q = 0; q < 3; ++q) {
-      fs[q] = open_in(d, names[q], "w");
-      if (!fs[q]) return set_error(FDTD2D_EIO, std::string("cannot write ") + names[q]);
-    }
-    fs[3] = open_in(d, "detP.txt", "w");
-    if (!fs[3]) return set_error(FDTD2D_EIO, "cannot write detP.txt");
-    const int rows = ncomp / (S.det_cols > 0 ? S.det_cols : 1);
-    const int cols = S.det_cols > 0 ? S.det_cols : 1;
-    for (int i = 0; i < nMax; ++i) {
-      for (int j = 0; j < rows; ++j) {
-        for (int k = 0; k < cols; ++k)
-          for (int q = 0; q < 3; ++q)
-            std::fprintf(fs[q], "%12lf\t", comp[q * plane + static_cast<size_t>(i) * ncomp + j * cols + k] * kDetMult);
-        for (int q = 0; q < 4; ++q) std::fprintf(fs[q], "\n");
-      }
-      for (int q = 0; q < 4; ++q) std::fprintf(fs[q], "\n");
-    }
-    for (int q = 0; q < 4; ++q) std::fclose(fs[q]);
+    if (int rc = write_components(S, d, "", comp, true)) return rc;
   }
   // SourceP.txt: the source envelope, one "%f" line per step index 0..nMax-1.
   {
@@ -122,7 +143,8 @@ extern "C" int fdtd2d_write_outputs(const fdtd2d_setup *setup, const char *dir, 
     std::fprintf(f,
                  "\nSAVE DATA PARAMETER\nSlises saved: %d\nData saved from time %d to time %d \nTime sieve "
                  "parameter: %d \nNet sieve parameter: %d\nData multiplier: %f\n",
-                 0, static_cast<int>(get("From")), static_cast<int>(get("To")), static_cast<int>(get("TimeSieve")),
+                 static_cast<int>(get("SlicesSaved")), static_cast<int>(get("From")), static_cast<int>(get("To")),
+                 static_cast<int>(get("TimeSieve")),
                  static_cast<int>(get("GridSieve")), 1.0);
     std::fprintf(f,
                  "\nSORCE AND DETECTOR\n Source point: (%d, %d, %d)\n Detector point(%d, %d, %d)\n Detector "
@@ -208,5 +230,60 @@ extern "C" int fdtd2d_write_rc_outputs(const fdtd2d_setup *setup, const char *di
       out << ft << " " << ft * p.dt * kTimePrint << std::endl;
     }
   }
+  return FDTD2D_OK;
+}
+
+extern "C" int fdtd2d_write_s2d_outputs(const fdtd2d_setup *setup, const char *dir, const float *det,
+                                        const float *comp) {
+  using fdtd2d::set_error;
+  if (!setup || !dir || !det || !comp) return set_error(FDTD2D_EINVAL, "null argument");
+  const fdtd2d::Setup &S = setup->s;
+  const fdtd2d_problem &p = S.p;
+  const std::string d(dir);
+  // Detect_S2D.txt: t [us] and the detector sum x 1000 (F/kernel.cu:1827-1834)
+  FILE *f = open_in(d, "Detect_S2D.txt", "w");
+  if (!f) return set_error(FDTD2D_EIO, "cannot write Detect_S2D.txt in " + d);
+  for (int i = 0; i < p.nsteps; ++i) {
+    double sum = 0;
+    for (int k = 0; k < p.ndet; ++k) sum += det[static_cast<size_t>(i) * p.ndet + k];
+    std::fprintf(f, "%lf\t%12lf\n ", (p.dt * i) * 1000000, sum * kDetMult);
+  }
+  std::fclose(f);
+  if (int rc = write_components(S, d, "_S2D", comp, false)) return rc;
+  for (const char *name : {"CompDetect_S2D.txt", "detEx_S2D.txt", "detEy_S2D.txt", "detHz_S2D.txt"})
+    if (int rc = touch(d, name)) return rc;
+  return FDTD2D_OK;
+}
+
+extern "C" int fdtd2d_write_s2d_difference(const fdtd2d_setup *setup, const char *dir, const float *comp,
+                                           const float *comp_free) {
+  using fdtd2d::set_error;
+  if (!setup || !dir || !comp || !comp_free) return set_error(FDTD2D_EINVAL, "null argument");
+  const fdtd2d_problem &p = setup->s.p;
+  const int nMax = p.nsteps, ncomp = p.ncomp;
+  const size_t plane = static_cast<size_t>(nMax + 1) * ncomp;
+  FILE *f = open_in(dir, "DifDetEx.txt", "w");
+  if (!f) return set_error(FDTD2D_EIO, std::string("cannot write DifDetEx.txt in ") + dir);
+  // F/kernel.cu:2497-2536: per element the float difference, per step the
+  // double sums of the Ez, Hx, Hy differences and the
+  // magnitude of E x H from float components (Ex, Ey, Hz sums are 0)
+  for (int i = 0; i < nMax; ++i) {
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};  // s[1..6] = detSum .. detSum6
+    for (int k = 0; k < ncomp; ++k) {
+      const size_t q = static_cast<size_t>(i) * ncomp + k;
+      for (int c = 0; c < 3; ++c) {
+        const float diff = comp[c * plane + q] - comp_free[c * plane + q];
+        s[3 + c] += diff;
+      }
+    }
+    const float eh_i = static_cast<float>(s[2] * s[6] - s[3] * s[5]);
+    const float eh_j = static_cast<float>(-(s[1] * s[6] - s[3] * s[4]));
+    const float eh_k = static_cast<float>(s[1] * s[5] - s[2] * s[4]);
+    const double mag = std::sqrt(std::pow(eh_i, 2) + std::pow(eh_j, 2) + std::pow(eh_k, 2));
+    std::fprintf(f, "%lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\t%12lf\n ", (p.dt * i) * 1000000,
+                 s[1] * kDetMult, s[2] * kDetMult, s[3] * kDetMult, s[4] * kDetMult, s[5] * kDetMult, s[6] * kDetMult,
+                 mag * kDetMult);
+  }
+  std::fclose(f);
   return FDTD2D_OK;
 }

@@ -204,9 +204,12 @@ class Setup:
         self._h = C.c_void_p(handle)
 
     @classmethod
-    def from_ini(cls, path: str) -> "Setup":
+    def from_ini(cls, path: str, objects: bool = True) -> "Setup":
+        """objects=False: the free-space pass of the source-to-detector regime
+        (no chamber figure, no plasma medium)."""
         h = C.c_void_p()
-        L.check(L.lib().fdtd2d_setup_from_ini(str(path).encode(), C.byref(h)), "fdtd2d_setup_from_ini")
+        L.check(L.lib().fdtd2d_setup_from_ini_objects(str(path).encode(), int(objects), C.byref(h)),
+                "fdtd2d_setup_from_ini_objects")
         return cls(h.value)
 
     @classmethod
@@ -236,6 +239,30 @@ class Setup:
         v = C.c_double()
         L.check(L.lib().fdtd2d_setup_get(self._h, name.encode(), C.byref(v)), f"fdtd2d_setup_get({name})")
         return v.value
+
+    def set(self, name: str, value: float) -> None:
+        L.check(L.lib().fdtd2d_setup_set(self._h, name.encode(), float(value)), f"fdtd2d_setup_set({name})")
+
+    def snapshot_steps(self, nsteps_run: int) -> np.ndarray:
+        """Steps after which [2Dsave] saves Ez, for a run of nsteps_run steps."""
+        n = C.c_size_t()
+        L.check(L.lib().fdtd2d_snapshot_steps(self._h, nsteps_run, L._i(), 0, C.byref(n)), "fdtd2d_snapshot_steps")
+        out = np.zeros(n.value, np.int32)
+        L.check(L.lib().fdtd2d_snapshot_steps(self._h, nsteps_run, _ip(out), out.size, C.byref(n)),
+                "fdtd2d_snapshot_steps")
+        return out
+
+    def write_s2d_outputs(self, out_dir: str, det: np.ndarray, comp: np.ndarray) -> None:
+        d = np.ascontiguousarray(det, np.float32)
+        c = np.ascontiguousarray(comp, np.float32)
+        L.check(L.lib().fdtd2d_write_s2d_outputs(self._h, str(out_dir).encode(), _fp(d), _fp(c)),
+                "fdtd2d_write_s2d_outputs")
+
+    def write_s2d_difference(self, out_dir: str, comp: np.ndarray, comp_free: np.ndarray) -> None:
+        c = np.ascontiguousarray(comp, np.float32)
+        c0 = np.ascontiguousarray(comp_free, np.float32)
+        L.check(L.lib().fdtd2d_write_s2d_difference(self._h, str(out_dir).encode(), _fp(c), _fp(c0)),
+                "fdtd2d_write_s2d_difference")
 
     def footprint(self) -> int:
         """Device bytes a plan (or pool shot) of this setup holds."""
@@ -269,6 +296,26 @@ class Setup:
             self.close()
         except Exception:
             pass
+
+
+class Snapshots:
+    """Ez.txt and its companion files of [2Dsave] (fdtd2d_snapshots_*)."""
+
+    def __init__(self, setup: Setup, out_dir: str):
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_snapshots_open(setup._h, str(out_dir).encode(), C.byref(h)), "fdtd2d_snapshots_open")
+        self._h = h
+
+    def add(self, ez: np.ndarray) -> None:
+        v = np.ascontiguousarray(ez, np.float32)
+        L.check(L.lib().fdtd2d_snapshots_add(self._h, _fp(v), v.size), "fdtd2d_snapshots_add")
+
+    def close(self) -> int:
+        n = C.c_int()
+        if self._h:
+            L.check(L.lib().fdtd2d_snapshots_close(self._h, C.byref(n)), "fdtd2d_snapshots_close")
+            self._h = C.c_void_p()
+        return n.value
 
 
 FIELD_SHAPES = {

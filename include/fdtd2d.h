@@ -173,6 +173,12 @@ int fdtd2d_device_count(int *count);
  * to the compile-time defaults, as the reference does (F/FDTD_cuda.cu:30-33);
  * fdtd2d_setup_ini_status reports what happened (0 ok, -1 open failed, >0 first bad line). */
 int fdtd2d_setup_from_ini(const char *ini_path, fdtd2d_setup **out);
+/* As fdtd2d_setup_from_ini with the reference's ObjectsEnable flag
+ * (F/FDTD_cuda.cu:389-419, 2602-2659): objects = 0 builds neither the chamber
+ * figure nor the plasma medium, the first (free-space) pass of the
+ * source-to-detector regime ([regimes] SourceToDetect = 1); objects = 1 is
+ * fdtd2d_setup_from_ini. */
+int fdtd2d_setup_from_ini_objects(const char *ini_path, int objects, fdtd2d_setup **out);
 int fdtd2d_setup_ini_status(const fdtd2d_setup *s);
 
 /* One Settings.ini value from INI text in memory, read as the reference's
@@ -203,6 +209,9 @@ fdtd2d_problem *fdtd2d_setup_problem_mut(fdtd2d_setup *s);
 /* Scalar value of a named setup parameter (e.g. "dt", "Imax", "nMax",
  * "fCoef", "qantI"); returns FDTD2D_EINVAL for unknown names. */
 int fdtd2d_setup_get(const fdtd2d_setup *s, const char *name, double *value);
+/* Sets a value the writers report: "SlicesSaved" (Ez snapshots written, the
+ * "Slises saved" line of log.txt). */
+int fdtd2d_setup_set(fdtd2d_setup *s, const char *name, double value);
 void fdtd2d_setup_destroy(fdtd2d_setup *s);
 
 /* Reference output files (Detect.txt, detAll.txt, detEz/Hx/Hy.txt,
@@ -240,6 +249,30 @@ int fdtd2d_processing_tof(const fdtd2d_processing *a, const float *line, int tim
 int fdtd2d_write_processing_outputs(const fdtd2d_processing *a, const char *dir, const float *line, int times);
 int fdtd2d_write_outputs(const fdtd2d_setup *s, const char *dir, const float *det,
                          const float *comp, double seconds);
+/* Source-to-detector regime ([regimes] SourceToDetect = 1, F/FDTD_cuda.cu:
+ * 389-419): the free-space pass's files (Detect_S2D.txt, detAll_S2D.txt,
+ * detEz/Hx/Hy_S2D.txt and the empty files the reference opens,
+ * DevSourceToDetectInfluenceCalc F/kernel.cu:1620-1926) from that pass's
+ * traces, and DifDetEx.txt (F/kernel.cu:2497-2536) from the component traces
+ * of the main pass minus those of the free-space pass. */
+int fdtd2d_write_s2d_outputs(const fdtd2d_setup *s, const char *dir, const float *det, const float *comp);
+int fdtd2d_write_s2d_difference(const fdtd2d_setup *s, const char *dir, const float *comp,
+                                const float *comp_free);
+
+/* Ez snapshots of [2Dsave] (F/kernel.cu:2173-2302, 3041-3051). The steps
+ * n <= nsteps_run after which the reference saves Ez: a counter starting at 1
+ * counts the steps inside [From, To] and saves when it equals TimeSieve
+ * (steps may be NULL to query *count). */
+int fdtd2d_snapshot_steps(const fdtd2d_setup *s, int nsteps_run, int32_t *steps, size_t capacity,
+                          size_t *count);
+/* Ez.txt (one "%12f " line of Ez down each sieved column per snapshot), the
+ * newline-only Ex/Hx/Hy/Hz.txt, the empty Ey.txt, and on close
+ * FieldSaveDat.txt ("%d %d": lines per snapshot, snapshots). ez is laid out
+ * as fdtd2d_read_field returns it. */
+typedef struct fdtd2d_snapshots fdtd2d_snapshots;
+int fdtd2d_snapshots_open(const fdtd2d_setup *s, const char *dir, fdtd2d_snapshots **out);
+int fdtd2d_snapshots_add(fdtd2d_snapshots *w, const float *ez, size_t count);
+int fdtd2d_snapshots_close(fdtd2d_snapshots *w, int *saved);
 
 /* ------------------------------------------------------------------ plans
  * Replaces DevMemoryAllocation / the DevFieldCalculation loop /

@@ -176,15 +176,6 @@ def test_missing_ini_falls_back_to_compile_time_defaults(tmp_path):
     assert (p.nx, p.ny, p.nsteps) == (550, 600, 10000)  # I_SIZE, J_SIZE, MAX_TIME_STEP with DS = 1
 
 
-def test_unsupported_geometry_is_reported(tmp_path):
-    text = (GOLD / "settings_release.ini").read_text().replace("Draw2D=0", "Draw2D=1")
-    f = tmp_path / "Settings.ini"
-    f.write_text(text)
-    with pytest.raises(F.FDTD2DError) as e:
-        F.Setup.from_ini(str(f))
-    assert e.value.code == 6
-
-
 def test_plain_workload_parameters():
     """400x400, 1 mm, dt = 0.99 x 2D Courant, CPML 12 with
     sigma_max = 0.8(m+1)/(eta0 dx), kappa_max = 1, linear ramp to 2 nc(10 GHz)
