@@ -457,7 +457,13 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
 // Levels 1..T of one row iteration: level t computes row k-t from level t-1.
 // EV: apply the sources and take the samples of the events on those rows
 // (only CPML-class tiles hold event cells).
-template <int T, int C, bool EDGE, int MED, bool JZ, bool EV>
+// WARM (the first iterations of a band): level t's rows are needed from row
+// jb0 - (T-t) - 1 on (the rows the stored rows depend on, plus one row of Hx
+// above them), i.e. from iteration k = jb0 - T - 1 + 2t; before that the
+// level and all later ones are skipped. A skipped level leaves its pipeline
+// register as it was; the first needed row of a level reads only its Hx
+// from that register's rows, and only for an Ez nothing needs.
+template <int T, int C, bool EDGE, int MED, bool JZ, bool EV, bool WARM>
 __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl, int n0,
                                        int k, int jb0, int jb1, int own0, int own1, Level<C, EDGE> prev,
                                        Level<C, EDGE> (&st)[T + 1]) {
@@ -466,6 +472,9 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
   for (int c = 0; c < C; ++c) nxt_ez[c] = st[0].ez[c];
 #pragma unroll
   for (int t = 1; t <= T; ++t) {
+    if constexpr (WARM) {
+      if (k < jb0 - T - 1 + 2 * t) break;
+    }
     const int r = k - t;
     Level<C, EDGE> out;
     const bool own_row = r >= jb0 && r < jb1;
@@ -480,7 +489,7 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
 
 // One row iteration: level 0 takes row k, level t computes row k-t, and
 // row k-T is stored.
-template <int T, int C, bool EDGE, int MED, bool JZ>
+template <int T, int C, bool EDGE, int MED, bool JZ, bool WARM>
 __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, const float4 *lut_s,
                                          const Cols<C, EDGE> &cl, int src, int n0, int k, int jb0, int jb1,
                                          int own0, int own1, bool lane_owns, unsigned trash_off,
@@ -488,7 +497,7 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
   Level<C, EDGE> prev = st[0];
   st[0] = in;
   // ---- levels 1..T, each one row behind the previous
-  levels<T, C, EDGE, MED, JZ, EDGE>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st);
+  levels<T, C, EDGE, MED, JZ, EDGE, WARM>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st);
   // ---- store step n0+T for owned cells of row k-T
   const int r = k - T;
   if constexpr (!EDGE) {
@@ -595,17 +604,32 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
     const int kk = EDGE ? jl0 + u : min(jl0 + u, jl1 - 1);
     load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, jl0 + u < jl1, lane_in, pre[u]);
   }
-  // Iterations k in [jl0, jl1 + T), rounded up to a multiple of D: the extra
-  // rows lie below the band's output rows and store nothing.
-  for (int k = jl0; k < jl1 + T; k += D) {
+  // Iterations k in [jl0, jb1 + T): the band's last row is stored at
+  // k = jb1 - 1 + T. The first ones, up to k = jb0 + T - 2, run only the
+  // levels whose rows are needed (WARM); the count is rounded up to a
+  // multiple of D, and rows past the end store nothing.
+  const int kwarm = jb0 + T - 1, kend = jb1 + T;
+  int k = jl0;
+  for (; k < kwarm; k += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const Level<C, EDGE> in = pre[u];
       const int kn = k + u + D;
       const int kk = EDGE ? kn : min(kn, jl1 - 1);
       load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
-      row_step<T, C, EDGE, MED, JZ>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns, trash_off, in,
-                                    st);
+      row_step<T, C, EDGE, MED, JZ, true>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
+                                          trash_off, in, st);
+    }
+  }
+  for (; k < kend; k += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const Level<C, EDGE> in = pre[u];
+      const int kn = k + u + D;
+      const int kk = EDGE ? kn : min(kn, jl1 - 1);
+      load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
+      row_step<T, C, EDGE, MED, JZ, false>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
+                                           trash_off, in, st);
     }
   }
 }
@@ -643,7 +667,7 @@ __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a
 typedef const StepArgs __attribute__((address_space(4))) *cargs_ptr;
 
 template <int T, int C, int D, int MED, bool EDGE>
-__global__ void __launch_bounds__(64 * kWavesPerWG, EDGE ? 4 : 5)
+__global__ void __launch_bounds__(64 * kWavesPerWG, 4)
     pool_kernel(const StepArgs *__restrict__ descs, PoolQueue *q, int k, int tp) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   __shared__ int s_pos;
