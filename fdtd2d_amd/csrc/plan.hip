@@ -654,6 +654,8 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   a.jz_decay = p->jz_decay;
   a.dxf = p->dx_f;
   a.dyf = p->dy_f;
+  a.rdx = 1.0 / static_cast<double>(p->dx_f);
+  a.rdy = 1.0 / static_cast<double>(p->dy_f);
   a.jz_dt = p->jz_dt;
   a.uden_hx = u_hx;
   a.uden_ex = u_ex;

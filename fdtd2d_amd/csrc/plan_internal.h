@@ -35,6 +35,7 @@ struct StepArgs {
   int pec;             // TABLE: material forced to Ez = 0 (-1 none)
   float db, ca0, cb0, cj, jz_decay, dxf, dyf;
   double jz_dt;
+  double rdx, rdy;  // the doubles nearest 1/dxf and 1/dyf
   // fields [2 buffers]
   float *ez[2], *hx[2], *hy[2], *jz[2];
   // psi [2 buffers]: y-strips [2P rows][pitch] by slot; x-strips [ny][xs] by slot
