@@ -2,8 +2,9 @@
 launch-shape autotuner.
 
 The autotuner tunes a live pool by coordinate descent over the launch shape
-(steps per launch, band rows, CPML sub-band rows, persistent work-groups per
-CU of each kernel, prefetch depth). Every trial advances the real shots -
+(one fused kernel or two per launch, steps per launch, band rows, CPML
+sub-band rows, persistent work-groups per CU of each kernel, prefetch
+depth). Every trial advances the real shots -
 results are bitwise independent of the shape - and is scored by the cell
 updates it completed per second of wall time, measured with the pool synced
 on both sides.
@@ -21,9 +22,9 @@ from .problem import Pool, Setup
 TUNE_AXES = [
     ("tblock", [8, 6, 7]),
     ("band", [128, 64, 256]),
-    ("wg_per_cu", [4, 3, 5]),
+    ("wg_per_cu", [4, 3]),
     ("band_cpml", [32, 16, 64]),
-    ("wg_per_cu_cpml", [1, 2]),
+    ("wg_per_cu_cpml", [1, 2, 3]),
     ("prefetch", [2, 1]),
 ]
 
@@ -45,27 +46,45 @@ def _trial(pool: Pool, launches: int) -> float:
     return (pool.info("cell_updates") - c0) / dt if dt > 0 else 0.0
 
 
-def autotune(pool: Pool, launches_per_trial: int = 4, axes=TUNE_AXES, log=None) -> dict:
-    """Picks the launch shape for the shots now in the pool; returns it with
-    the score of every trial. Stops early when the pool runs out of work."""
-    best = {name: values[0] for name, values in axes}
+def _descend(pool: Pool, start: dict, axes, launches_per_trial: int, trials: list, log=None):
+    best = dict(start)
     for name, value in best.items():
         pool.set_option(name, value)
-    trials = []
     best_rate = _trial(pool, launches_per_trial)
     trials.append((dict(best), best_rate))
     for name, values in axes:
-        for v in values[1:]:
-            if pool.info("running") == 0:
-                break
+        for v in values:
+            if v == best[name] or pool.info("running") == 0:
+                continue
             pool.set_option(name, v)
             rate = _trial(pool, launches_per_trial)
             trials.append(({**best, name: v}, rate))
             if log:
-                log(f"autotune {name}={v}: {rate / 1e9:.1f} Gcell/s (best {best_rate / 1e9:.1f})")
+                log(f"autotune fused={best.get('fused')} {name}={v}: {rate / 1e9:.1f} Gcell/s "
+                    f"(best {best_rate / 1e9:.1f})")
             if rate > best_rate:
                 best_rate, best[name] = rate, v
         pool.set_option(name, best[name])
+    return best, best_rate
+
+
+def autotune(pool: Pool, launches_per_trial: int = 4, axes=TUNE_AXES, log=None) -> dict:
+    """Picks the launch shape for the shots now in the pool; returns it with
+    the score of every trial. One coordinate descent per kernel arrangement
+    (one fused kernel per launch, or one kernel per tile class), the better
+    one kept. Stops early when the pool runs out of work."""
+    trials = []
+    found = []
+    for fused in (1, 0):
+        if pool.info("running") == 0:
+            break
+        start = {"fused": fused, **{name: values[0] for name, values in axes}}
+        found.append(_descend(pool, start, axes, launches_per_trial, trials, log))
+    if not found:
+        return {"shape": {}, "rate": 0.0, "trials": trials}
+    best, best_rate = max(found, key=lambda f: f[1])
+    for name, value in best.items():
+        pool.set_option(name, value)
     return {"shape": best, "rate": best_rate, "trials": trials}
 
 

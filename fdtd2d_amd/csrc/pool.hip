@@ -130,6 +130,7 @@ struct fdtd2d_pool {
   // launch shape
   int T = 8, band = 0, band_cpml = 0, D = 2;
   int wg_per_cu = 4, wg_per_cu_cpml = 1;
+  int fused = 1;  // one kernel per launch, CPML-class work first (0: two kernels on two streams)
   int chunk = 8;  // launches per captured graph; 0 = no graphs
   int medium_kind = -1;
   // memory
@@ -246,6 +247,13 @@ int record_done(fdtd2d_pool *pp) {
 // One pool launch of the main lists at chunk position k (both classes).
 int launch_main(fdtd2d_pool *pp, int k) {
   const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
+  if (pp->fused) {
+    // one kernel takes the CPML-class work-groups first, then the interior ones
+    hipError_t e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 2, pp->T, pp->D, pp->medium_kind, gi,
+                                       pp->stream);
+    if (e != hipSuccess) return hip_fail(e, "pool kernel launch");
+    return FDTD2D_OK;
+  }
   HIP_TRY(hipEventRecord(pp->ev_fork, pp->stream), "hipEventRecord fork");
   HIP_TRY(hipStreamWaitEvent(pp->aux, pp->ev_fork, 0), "hipStreamWaitEvent fork");
   hipError_t e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 1, pp->T, 1, pp->medium_kind, gc, pp->aux);
@@ -309,11 +317,18 @@ int run_tails(fdtd2d_pool *pp, const std::vector<int> &tails) {
     const int n0 = static_cast<int>(lists[0].size() / 4), n1 = static_cast<int>(lists[1].size() / 4);
     const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
     hipError_t e = fdtd2d::launch_pool_set(pp->d_qt, pp->d_tail[0], pp->d_tail[1], n0, n1, pp->g, pp->stream);
-    if (e == hipSuccess && n1 > 0)
-      e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 1, t, 1, pp->medium_kind, std::min(gc, n1), pp->stream);
-    if (e == hipSuccess && n0 > 0)
-      e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 0, t, pp->D, pp->medium_kind, std::min(gi, n0),
-                              pp->stream);
+    if (pp->fused) {
+      if (e == hipSuccess && n0 + n1 > 0)
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 2, t, pp->D, pp->medium_kind, std::min(gi, n0 + n1),
+                                pp->stream);
+    } else {
+      if (e == hipSuccess && n1 > 0)
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 1, t, 1, pp->medium_kind, std::min(gc, n1),
+                                pp->stream);
+      if (e == hipSuccess && n0 > 0)
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 0, t, pp->D, pp->medium_kind, std::min(gi, n0),
+                                pp->stream);
+    }
     if (e == hipSuccess) e = fdtd2d::launch_pool_tick(pp->d_qt, 1, 0, pp->stream);
     if (e != hipSuccess) return hip_fail(e, "pool tail launch");
     pp->tail_launches++;
@@ -504,6 +519,10 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
     if (v != 1 && v != 2) return set_error(FDTD2D_EINVAL, "prefetch must be 1 or 2");
     pp->D = static_cast<int>(v);
     pp->graph_valid = false;
+  } else if (n == "fused") {
+    if (v != 0 && v != 1) return set_error(FDTD2D_EINVAL, "fused must be 0 or 1");
+    pp->fused = static_cast<int>(v);
+    pp->graph_valid = false;
   } else if (n == "wg_per_cu") {
     if (v < 1 || v > 8) return set_error(FDTD2D_EINVAL, "wg_per_cu must be 1..8");
     pp->wg_per_cu = static_cast<int>(v);
@@ -539,6 +558,7 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value)
   else if (n == "band_cpml") *value = pp->band_cpml;
   else if (n == "prefetch") *value = pp->D;
   else if (n == "wg_per_cu") *value = pp->wg_per_cu;
+  else if (n == "fused") *value = pp->fused;
   else if (n == "wg_per_cu_cpml") *value = pp->wg_per_cu_cpml;
   else if (n == "graph_chunk") *value = pp->chunk;
   else if (n == "budget_bytes") *value = static_cast<double>(pp->budget);

@@ -678,37 +678,48 @@ __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a
 // StepArgs are read through the constant address space (scalar loads).
 typedef const StepArgs __attribute__((address_space(4))) *cargs_ptr;
 
-template <int T, int C, int D, int MED, bool EDGE>
+// MODE 0: interior tiles, 1: CPML-class tiles, 2: both, the CPML-class
+// ones first (one kernel for a whole launch: every wave takes CPML-class work
+// while there is any, so the longer CPML-class tiles never run at the tail
+// with the rest of the device idle).
+template <int T, int C, int D, int MED, int MODE>
 __global__ void __launch_bounds__(64 * kWavesPerWG, 4)
     pool_kernel(const StepArgs *__restrict__ descs, PoolQueue *q, int k, int tp) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
-  __shared__ int s_pos;
-  constexpr int cls = EDGE ? 1 : 0;
-  const int4 *items = q->items[cls];
-  const int count = __builtin_amdgcn_readfirstlane(q->count[cls]);
+  __shared__ int s_pos, s_cls;
   const int g = __builtin_amdgcn_readfirstlane(q->g) + k;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int x = blockIdx.x & 7, tried = 0;  // used by thread 0 only
+  int x = blockIdx.x & 7, tried = 0, cls_t = MODE == 0 ? 0 : 1;  // used by thread 0 only
   int lut_slot = -1;
   for (;;) {
     if (threadIdx.x == 0) {
       int pos = -1;
-      while (tried < 8) {
-        const int lo = static_cast<int>(static_cast<long long>(x) * count / 8);
-        const int hi = static_cast<int>(static_cast<long long>(x + 1) * count / 8);
-        const int t = atomicAdd(&q->ticket[k][cls][x], 1);
-        if (lo + t < hi) {
-          pos = lo + t;
-          break;
+      for (;;) {
+        const int count = q->count[cls_t];
+        while (tried < 8) {
+          const int lo = static_cast<int>(static_cast<long long>(x) * count / 8);
+          const int hi = static_cast<int>(static_cast<long long>(x + 1) * count / 8);
+          const int t = atomicAdd(&q->ticket[k][cls_t][x], 1);
+          if (lo + t < hi) {
+            pos = lo + t;
+            break;
+          }
+          x = (x + 1) & 7;
+          ++tried;
         }
-        x = (x + 1) & 7;
-        ++tried;
+        if (pos >= 0 || MODE != 2 || cls_t == 0) break;
+        cls_t = 0;  // the CPML-class list is drained: go on with the interior one
+        x = blockIdx.x & 7;
+        tried = 0;
       }
       s_pos = pos;
+      s_cls = cls_t;
     }
     __syncthreads();
     const int pos = __builtin_amdgcn_readfirstlane(s_pos);
+    const int cls = __builtin_amdgcn_readfirstlane(s_cls);
     if (pos < 0) return;  // every range is exhausted (uniform over the work-group)
+    const int4 *items = q->items[cls];
     const long long e = 4LL * (4LL * pos + wave);
     const int slot = cld_i(items, 4LL * 4LL * pos + 3) >> 8;
     const StepArgs &a = *(const StepArgs *)((cargs_ptr)(descs + slot));
@@ -724,10 +735,21 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, 4)
     const int n0 = a.n_base + dg * tp;
     const int src = a.src_base ^ (dg & 1);
     if (r0 < r1) {
-      if (EDGE || (w & 1))
-        run_tile<T, C, D, EDGE, MED, true>(a, lut_s, src, n0, tx, r0, r1);
-      else
-        run_tile<T, C, D, EDGE, MED, false>(a, lut_s, src, n0, tx, r0, r1);
+      if constexpr (MODE == 1) {
+        run_tile<T, C, 1, true, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+      } else {
+        bool edge_tile = false;
+        if constexpr (MODE == 2) edge_tile = cls == 1;
+        if constexpr (MODE == 2) {
+          if (edge_tile) run_tile<T, C, 1, true, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+        }
+        if (!edge_tile) {
+          if (w & 1)
+            run_tile<T, C, D, false, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+          else
+            run_tile<T, C, D, false, MED, false>(a, lut_s, src, n0, tx, r0, r1);
+        }
+      }
     }
     // every wave is done with s_pos and the LUT before they are rewritten
     __syncthreads();
@@ -816,29 +838,29 @@ __global__ void __launch_bounds__(64) pool_set(PoolQueue *q, const int4 *items0,
   }
 }
 
-template <int T, int D, bool EDGE>
+template <int T, int D, int MODE>
 hipError_t launch_pool_t(const StepArgs *descs, PoolQueue *q, int k, int tp, int med, int grid, hipStream_t s) {
   if (med == kTable)
-    hipLaunchKernelGGL((pool_kernel<T, 1, D, kTable, EDGE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q, k,
+    hipLaunchKernelGGL((pool_kernel<T, 1, D, kTable, MODE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q, k,
                        tp);
   else
-    hipLaunchKernelGGL((pool_kernel<T, 1, D, kDensity, EDGE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q,
+    hipLaunchKernelGGL((pool_kernel<T, 1, D, kDensity, MODE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q,
                        k, tp);
   return hipGetLastError();
 }
 
-template <int D, bool EDGE>
+template <int D, int MODE>
 hipError_t launch_pool_d(const StepArgs *descs, PoolQueue *q, int k, int tp, int T, int med, int grid,
                          hipStream_t s) {
   switch (T) {
-    case 1: return launch_pool_t<1, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 2: return launch_pool_t<2, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 3: return launch_pool_t<3, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 4: return launch_pool_t<4, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 5: return launch_pool_t<5, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 6: return launch_pool_t<6, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 7: return launch_pool_t<7, D, EDGE>(descs, q, k, tp, med, grid, s);
-    case 8: return launch_pool_t<8, D, EDGE>(descs, q, k, tp, med, grid, s);
+    case 1: return launch_pool_t<1, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 2: return launch_pool_t<2, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 3: return launch_pool_t<3, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 4: return launch_pool_t<4, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 5: return launch_pool_t<5, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 6: return launch_pool_t<6, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 7: return launch_pool_t<7, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 8: return launch_pool_t<8, D, MODE>(descs, q, k, tp, med, grid, s);
   }
   return hipErrorInvalidValue;
 }
@@ -874,10 +896,14 @@ hipError_t launch_cd(const StepArgs &a, int T, int src, int n0, hipStream_t s) {
 
 hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D, int medium_kind,
                        int grid, hipStream_t stream) {
-  if (grid <= 0 || k < 0 || k >= kMaxChunk) return hipErrorInvalidValue;
-  if (cls != 0) return launch_pool_d<1, true>(descs, q, k, tp, T, medium_kind, grid, stream);
-  if (D == 2) return launch_pool_d<2, false>(descs, q, k, tp, T, medium_kind, grid, stream);
-  return launch_pool_d<1, false>(descs, q, k, tp, T, medium_kind, grid, stream);
+  if (grid <= 0 || k < 0 || k >= kMaxChunk || cls < 0 || cls > 2) return hipErrorInvalidValue;
+  if (cls == 1) return launch_pool_d<1, 1>(descs, q, k, tp, T, medium_kind, grid, stream);
+  if (cls == 2) {
+    if (D == 2) return launch_pool_d<2, 2>(descs, q, k, tp, T, medium_kind, grid, stream);
+    return launch_pool_d<1, 2>(descs, q, k, tp, T, medium_kind, grid, stream);
+  }
+  if (D == 2) return launch_pool_d<2, 0>(descs, q, k, tp, T, medium_kind, grid, stream);
+  return launch_pool_d<1, 0>(descs, q, k, tp, T, medium_kind, grid, stream);
 }
 
 hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,

@@ -352,10 +352,13 @@ int fdtd2d_plan_kernel_ms(fdtd2d_plan *plan, double *avg_ms, int *count);
  * ends inside them). Shots can be added and removed between steps.
  *
  * Options (fdtd2d_pool_set_option): "tblock" (1..8, default 8), "band",
- * "band_cpml", "prefetch" (1 or 2), "wg_per_cu" / "wg_per_cu_cpml"
- * (persistent work-groups per CU of each kernel), "graph_chunk" (0..16, 0 =
- * no graphs), "budget_bytes" (before any shot: carve shots from one arena of
- * that size and refuse shots that do not fit, FDTD2D_ENOMEM).
+ * "band_cpml", "prefetch" (1 or 2), "fused" (1, the default: one persistent
+ * kernel per launch that takes the CPML-class tiles first, then the interior
+ * ones; 0: one kernel per class on two streams), "wg_per_cu" /
+ * "wg_per_cu_cpml" (persistent work-groups per CU of the interior or fused
+ * kernel / of the CPML-class kernel), "graph_chunk" (0..16, 0 = no graphs),
+ * "budget_bytes" (before any shot: carve shots from one arena of that size
+ * and refuse shots that do not fit, FDTD2D_ENOMEM).
  * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",
  * "shots", "running", "launches", "cell_updates", "graph_replays",
  * "eager_launches", "tail_launches", "cus". */
