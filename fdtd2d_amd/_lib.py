@@ -229,8 +229,9 @@ def tile_lists(nx: int, ny: int, npml: int, tblock: int = 0, cols_per_lane: int 
     """(entries, geometry): the tile partition a plan of this shape launches.
 
     events are (i, j) cells of sources, detectors and probes. entries is a
-    list of (tile, first row, end row, class), geometry the dict
-    {vw, halo, band, band_cpml}. Host-only; needs no GPU."""
+    list of (first column, end column, first row, end row, class), geometry
+    the dict {vw, halo, band, band_cpml} of the interior tiles. Host-only;
+    needs no GPU."""
     n = C.c_size_t()
     g = (C.c_int * 4)()
     L = lib()
@@ -239,9 +240,9 @@ def tile_lists(nx: int, ny: int, npml: int, tblock: int = 0, cols_per_lane: int 
     ej = (C.c_int * max(ne, 1))(*[e[1] for e in events])
     args = (nx, ny, npml, tblock, cols_per_lane, band, band_cpml, ei, ej, ne)
     check(L.fdtd2d_tile_lists(*args, None, 0, C.byref(n), g), "fdtd2d_tile_lists")
-    buf = (C.c_int * (4 * max(n.value, 1)))()
+    buf = (C.c_int * (5 * max(n.value, 1)))()
     check(L.fdtd2d_tile_lists(*args, buf, n.value, C.byref(n), g), "fdtd2d_tile_lists")
-    ent = [tuple(buf[4 * k:4 * k + 4]) for k in range(n.value)]
+    ent = [tuple(buf[5 * k:5 * k + 5]) for k in range(n.value)]
     return ent, {"vw": g[0], "halo": g[1], "band": g[2], "band_cpml": g[3]}
 
 

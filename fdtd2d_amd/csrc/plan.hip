@@ -145,6 +145,8 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
   t.H = band > 0 ? std::max(band, T) : pick_band(nx, ny, t.vw, T);
   t.nty = (ny + t.H - 1) / t.H;
   t.hc = band_cpml > 0 ? std::max(band_cpml, T) : std::max(32, T);
+  t.halo_e = T;
+  t.vw_e = 64 - 2 * t.halo_e;
   // window of rows / columns a tile loads
   auto window = [&](int tx, int r0, int r1, int &jl0, int &jl1, int &cs, int &ce) {
     jl0 = std::max(0, r0 - T);
@@ -194,7 +196,9 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
         if (touches(tx, r0, r1)) {
           if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, r0, 0));
           run0 = -1;
-          lists[1].push_back(make_int4(tx, r0, r1, 0));
+          const int c1 = std::min(nx, tx * t.vw + t.vw);
+          for (int c0 = tx * t.vw; c0 < c1; c0 += t.vw_e)
+            lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, c1 - c0)), r0, r1, 0));
         } else if (run0 < 0) {
           run0 = r0;
         }
@@ -205,16 +209,20 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
   // of a work-group get neighbouring tiles and consecutive work-groups go down
   // the same columns. Pad each list to whole work-groups with empty entries.
   for (int c = 0; c < 2; ++c) {
-    std::stable_sort(lists[c].begin(), lists[c].end(), [](const int4 &u, const int4 &v) {
-      const int gu = u.x / fdtd2d::kWavesPerWG, gv = v.x / fdtd2d::kWavesPerWG;
+    // column-group index of an entry: tile / 4, or first column / (4 vw_e)
+    auto group = [&](const int4 &u) {
+      return c == 0 ? u.x / fdtd2d::kWavesPerWG : edge_entry_c0(u.x) / (fdtd2d::kWavesPerWG * t.vw_e);
+    };
+    std::stable_sort(lists[c].begin(), lists[c].end(), [&](const int4 &u, const int4 &v) {
+      const int gu = group(u), gv = group(v);
       if (gu != gv) return gu < gv;
       if (u.y != v.y) return u.y < v.y;
       return u.x < v.x;
     });
     for (int4 &w : lists[c]) {
       w.w = c == 0 ? needs_jz(w) : 1;
-      const long long c0 = static_cast<long long>(w.x) * t.vw;
-      const long long c1 = std::min<long long>(nx, c0 + t.vw);
+      const long long c0 = c == 0 ? static_cast<long long>(w.x) * t.vw : edge_entry_c0(w.x);
+      const long long c1 = c == 0 ? std::min<long long>(nx, c0 + t.vw) : c0 + edge_entry_w(w.x);
       t.cells[c] += static_cast<double>(std::max<long long>(0, c1 - c0)) * (w.z - w.y);
     }
     t.n_tiles[c] = static_cast<int>(lists[c].size());
@@ -240,6 +248,8 @@ int set_tiling(fdtd2d_plan *pl) {
   fdtd2d::StepArgs &a = pl->a;
   a.halo = t.halo;
   a.vw = t.vw;
+  a.halo_e = t.halo_e;
+  a.vw_e = t.vw_e;
   a.trash_slots = std::min(kTrashSlots, kTrashRows * pl->L.pitch / (64 * pl->C));
   a.ntx = t.ntx;
   a.H = t.H;
@@ -358,10 +368,14 @@ int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, i
   for (int c = 0; c < 2; ++c)
     for (int q = 0; q < t.n_tiles[c]; ++q, ++k) {
       const int4 &w = t.lists[c][q];
-      entries[4 * k] = w.x;
-      entries[4 * k + 1] = w.y;
-      entries[4 * k + 2] = w.z;
-      entries[4 * k + 3] = c;
+      const int c0 = c == 0 ? w.x * t.vw : edge_entry_c0(w.x);
+      const int c1 = c == 0 ? std::min(nx, c0 + t.vw) : c0 + edge_entry_w(w.x);
+      int *e = entries + 5 * k;
+      e[0] = c0;
+      e[1] = c1;
+      e[2] = w.y;
+      e[3] = w.z;
+      e[4] = c;
     }
   return FDTD2D_OK;
 }

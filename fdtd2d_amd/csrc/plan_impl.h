@@ -46,13 +46,20 @@ struct Layout {
   size_t total;
 };
 
-// Tile lists of one launch shape, one entry {tx, r0, r1, 0} per wave:
-// [0] interior, [1] CPML class. A (tile, band) whose loads touch a CPML strip
-// or the outer ring is cut into sub-bands of hc rows; the sub-bands that still
-// touch go to the CPML list, runs of the others are merged back into one
-// interior entry. Host-only, so the partition can be checked without a GPU.
+// Tile lists of one launch shape, one entry per wave: [0] interior
+// {tx, r0, r1, flags}, [1] CPML class {edge_entry_x(c0, w), r0, r1, flags}.
+// A (tile, band) whose loads touch a CPML strip or the outer ring is cut into
+// sub-bands of hc rows; the sub-bands that still touch go to the CPML list,
+// split into pieces of vw_e columns, runs of the others are merged back into
+// one interior entry. Host-only, so the partition can be checked without a
+// GPU.
+inline int edge_entry_x(int c0, int w) { return c0 | (w << kEdgeColBits); }
+inline int edge_entry_c0(int x) { return x & ((1 << kEdgeColBits) - 1); }
+inline int edge_entry_w(int x) { return x >> kEdgeColBits; }
+
 struct Tiling {
   int T = 0, halo = 0, vw = 0, ntx = 0, H = 0, nty = 0, hc = 0;
+  int halo_e = 0, vw_e = 0;  // CPML-class waves: 1 column per lane
   std::vector<int4> lists[2];  // padded to whole work-groups
   int n_tiles[2] = {0, 0};
   int n_nojz = 0;  // interior tiles that skip the Jz update

@@ -81,6 +81,7 @@ struct StepArgs {
   // tiling (fixed per plan; launches of fewer steps reuse the same tiles)
   int vw;        // stored columns per tile = 64*C - 2*halo
   int halo;      // recomputed columns on each side, >= steps per launch
+  int vw_e, halo_e;  // the same for CPML-class waves (1 column per lane)
   int H;         // rows per band
   int ntx, nty;  // tiles across / down
   // Tile lists, one entry per wave = {tile, first row, end row, flags}; wave
@@ -117,6 +118,10 @@ constexpr int kRowRec = 12;  // ints per row record
 
 // One launch advances T steps (1..kMaxT).
 constexpr int kMaxT = 8;
+// CPML-class list entries run one column per lane whatever the interior
+// kernel's columns per lane: their first field packs the first stored column
+// (low kEdgeColBits bits) and the stored width (above).
+constexpr int kEdgeColBits = 20;
 constexpr int kWavesPerWG = 4;
 
 // Launches the step kernel of work-group class `cls` (0 interior, 1 CPML)

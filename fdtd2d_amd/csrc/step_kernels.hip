@@ -551,17 +551,26 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
 // D = rows fetched ahead of the one being consumed (memory-level
 // parallelism per wave); the row loop is unrolled by D so the prefetch ring
 // is indexed statically.
+// x: the interior entry's tile index (columns [x*vw, x*vw+vw)), or the
+// CPML-class entry's packed first column and width (one column per lane).
 template <int T, int C, int D, bool EDGE, int MED, bool JZ>
 __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s, int src, int n0,
-                                         int tx, int jb0, int jb1) {
-  const int HALO = a.halo;  // >= T, a multiple of C
-  const int VW = a.vw;      // 64*C - 2*HALO
+                                         int x, int jb0, int jb1) {
+  static_assert(!EDGE || C == 1, "CPML-class tiles run one column per lane");
   const int lane = threadIdx.x & 63;
   const int nx = a.nx, ny = a.ny, pitch = a.pitch;
-  const int cs = tx * VW - HALO;
+  int own0, own1, cs;
+  if constexpr (EDGE) {
+    own0 = x & ((1 << kEdgeColBits) - 1);
+    own1 = min(nx, own0 + (x >> kEdgeColBits));
+    cs = own0 - a.halo_e;  // halo_e >= T
+  } else {
+    own0 = x * a.vw;  // vw = 64*C - 2*halo, halo >= T and a multiple of C
+    own1 = min(nx, own0 + a.vw);
+    cs = own0 - a.halo;
+  }
   Cols<C, EDGE> cl;
   cl.c0 = cs + lane * C;
-  const int own0 = tx * VW, own1 = min(nx, own0 + VW);
   const int jl0 = max(0, jb0 - T), jl1 = min(ny, jb1 + T);
   const bool lane_in = cl.c0 >= 0 && cl.c0 < pitch;  // the lane's C columns are in memory
   const bool lane_owns = cl.c0 >= own0 && cl.c0 + C <= own1;
@@ -619,8 +628,10 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
   // Iterations k in [jl0, jb1 + T): the band's last row is stored at
   // k = jb1 - 1 + T. The first ones, up to k = jb0 + T - 2, run only the
   // levels whose rows are needed (WARM); the count is rounded up to a
-  // multiple of D, and rows past the end store nothing.
-  const int kwarm = jb0 + T - 1, kend = jb1 + T;
+  // multiple of D, and rows past the end store nothing. With more than one
+  // column per lane a separate warm-up loop would double the live state the
+  // compiler allocates for (occupancy 2); there one loop checks every level.
+  const int kwarm = C > 1 ? jl0 : jb0 + T - 1, kend = jb1 + T;
   int k = jl0;
   for (; k < kwarm; k += D) {
 #pragma unroll
@@ -640,8 +651,8 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       const int kn = k + u + D;
       const int kk = EDGE ? kn : min(kn, jl1 - 1);
       load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
-      row_step<T, C, EDGE, MED, JZ, false>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
-                                           trash_off, in, st);
+      row_step<T, C, EDGE, MED, JZ, (C > 1)>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
+                                             trash_off, in, st);
     }
   }
 }
@@ -926,16 +937,10 @@ hipError_t launch_pool_tick(PoolQueue *q, int nk, int dg, hipStream_t stream) {
 }
 
 hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream) {
-  // Edge work-groups always use the shallow prefetch (their state already
-  // fills the register file); interior ones take the requested depth.
-  if (cls != 0) {
-    switch (C) {
-      case 1: return launch_cd<1, 1, true>(a, T, src, n0, stream);
-      case 2: return launch_cd<2, 1, true>(a, T, src, n0, stream);
-      case 4: return launch_cd<4, 1, true>(a, T, src, n0, stream);
-    }
-    return hipErrorInvalidValue;
-  }
+  // CPML-class work-groups always run one column per lane with the shallow
+  // prefetch (their state already fills the register file); interior ones
+  // take the requested shape.
+  if (cls != 0) return launch_cd<1, 1, true>(a, T, src, n0, stream);
   switch (C * 16 + D) {
     case 1 * 16 + 1: return launch_cd<1, 1, false>(a, T, src, n0, stream);
     case 1 * 16 + 2: return launch_cd<1, 2, false>(a, T, src, n0, stream);

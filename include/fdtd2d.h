@@ -309,13 +309,15 @@ int fdtd2d_read_psi(fdtd2d_plan *plan, int which, float *out, size_t count);
 int fdtd2d_write_psi(fdtd2d_plan *plan, int which, const float *in, size_t count);
 /* The tile partition a plan of this shape launches, computed on the host (no
    device needed), for uniform den tables and the nev event cells (sources,
-   detectors, probes) at (ev_i[e], ev_j[e]): entries[4k..4k+3] = {tile, first
-   row, end row, class} for k < *count, class 0 interior kernel, 1 CPML-class
-   kernel; tile t stores columns [t*vw, t*vw+vw). geometry (4 ints, may be
-   null) = {vw, halo columns, rows per band, rows per CPML sub-band}. entries
-   may be null to query *count. tblock 0 = the default. No reference
-   counterpart: the reference launches one thread per cell
-   (F/kernel.cu:2027-2061). */
+   detectors, probes) at (ev_i[e], ev_j[e]): entries[5k..5k+4] = {first
+   column, end column, first row, end row, class} of the cells wave k stores,
+   for k < *count; class 0 interior kernel (cols_per_lane columns per lane),
+   1 CPML-class kernel (always one column per lane: a CPML-class tile of the
+   interior geometry is split into pieces of 64 - 2*tblock columns).
+   geometry (4 ints, may be null) = {vw, halo columns, rows per band, rows per
+   CPML sub-band} of the interior tiles. entries may be null to query *count.
+   tblock 0 = the default. No reference counterpart: the reference launches
+   one thread per cell (F/kernel.cu:2027-2061). */
 int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, int band, int band_cpml,
                       const int *ev_i, const int *ev_j, int nev, int *entries, size_t capacity, size_t *count,
                       int *geometry);

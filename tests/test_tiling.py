@@ -22,16 +22,26 @@ def check_partition(nx, ny, P, T, C, band, band_cpml):
     cover = np.zeros((ny, nx), np.int32)
     Pe = max(P, 1)
     cells = [0, 0]
-    for tx, r0, r1, cls in ent:
-        assert 0 <= r0 < r1 <= ny
-        c0, c1 = tx * vw, min(nx, tx * vw + vw)
-        assert c0 < nx
+    vw_e = 64 - 2 * Tn  # CPML-class waves: one column per lane, halo T
+    for c0, c1, r0, r1, cls in ent:
+        assert 0 <= r0 < r1 <= ny and 0 <= c0 < c1 <= nx
+        if cls == 0:
+            assert c0 % vw == 0 and c1 == min(nx, c0 + vw)
+        else:
+            assert c1 - c0 <= vw_e
         cover[r0:r1, c0:c1] += 1
         cells[cls] += (c1 - c0) * (r1 - r0)
-        cs, ce = c0 - halo, c0 + vw + halo  # columns the wave loads
+        h, w = (halo, vw) if cls == 0 else (Tn, vw_e)
+        cs, ce = c0 - h, c0 + w + h  # columns the wave loads
         jl0, jl1 = max(0, r0 - Tn), min(ny, r1 + Tn)  # rows the wave loads
         clear = cs > Pe and ce < nx - Pe - 1 and jl0 > Pe and jl1 < ny - Pe - 1
-        assert clear == (cls == 0), (tx, r0, r1, cls)
+        # the interior kernel carries no masks: its loads must stay clear
+        if cls == 0:
+            assert clear, (c0, r0, r1)
+        # with one column per lane the CPML-class pieces are whole tiles of
+        # the interior geometry, which are CPML class only when they touch
+        if C == 1 and cls == 1:
+            assert not clear, (c0, r0, r1)
     assert cover.min() == 1 and cover.max() == 1
     return ent, g, cells
 
@@ -60,7 +70,7 @@ def test_partition_8192_default():
     # only the tiles next to the strips take the CPML-class kernel
     assert cells[1] / (8192 * 8192) < 0.03
     ntx = -(-8192 // g["vw"])
-    mid = {tx for tx, r0, _, c in ent if c == 1 and 200 < r0 < 7900}
+    mid = {c0 // g["vw"] for c0, _, r0, _, c in ent if c == 1 and 200 < r0 < 7900}
     assert mid == {0, ntx - 1}
 
 
@@ -74,12 +84,11 @@ def test_event_cells_take_cpml_class():
         ent, g = _lib.tile_lists(nx, ny, 8, T, 1, 0, 0, events=ev)
         vw, halo = g["vw"], g["halo"]
         cover = np.zeros((ny, nx), np.int32)
-        for tx, r0, r1, cls in ent:
-            c0 = tx * vw
-            cover[r0:r1, c0:min(nx, c0 + vw)] += 1
+        for c0, c1, r0, r1, cls in ent:
+            cover[r0:r1, c0:c1] += 1
             if cls == 0:
                 for i, j in ev:
-                    assert not (c0 - halo <= i < c0 + vw + halo and r0 - T <= j < r1 + T), (i, j, tx, r0, r1)
+                    assert not (c0 - halo <= i < c0 + vw + halo and r0 - T <= j < r1 + T), (i, j, c0, r0, r1)
         assert cover.min() == 1 and cover.max() == 1
 
 
