@@ -158,21 +158,31 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
   t.T = T;
   t.halo = ((T + C - 1) / C) * C;
   t.vw = 64 * C - 2 * t.halo;
-  t.ntx = (nx + t.vw - 1) / t.vw;
+  t.halo_e = T;
+  t.vw_e = 64 - 2 * t.halo_e;
+  if (C == 1) {
+    t.x0 = 0;
+    t.ntx = (nx + t.vw - 1) / t.vw;
+  } else {
+    const int Pe = std::max(P, 1);
+    // the first multiple of vw_e at least Pe + halo + 2: a tile from x0 (or
+    // ending x0 before nx) loads no column of the strips or the ring
+    t.x0 = t.vw_e * ((Pe + t.halo + 1 + t.vw_e) / t.vw_e);
+    t.ntx = std::max(0, (nx - 2 * t.x0) / t.vw);
+  }
+  const int xr = t.x0 + t.ntx * t.vw;  // right margin [xr, nx) (C > 1)
   t.H = band > 0 ? std::max(band, T) : pick_band(nx, ny, t.vw, T, C);
   t.nty = (ny + t.H - 1) / t.H;
   t.hc = band_cpml > 0 ? std::max(band_cpml, T) : std::max(32, T);
-  t.halo_e = T;
-  t.vw_e = 64 - 2 * t.halo_e;
   // window of rows / columns a tile loads
   auto window = [&](int tx, int r0, int r1, int &jl0, int &jl1, int &cs, int &ce) {
     jl0 = std::max(0, r0 - T);
     jl1 = std::min(ny, r1 + T);
-    cs = std::max(0, tx * t.vw - t.halo);
-    ce = std::min(nx, tx * t.vw + t.vw + t.halo);
+    cs = std::max(0, t.x0 + tx * t.vw - t.halo);
+    ce = std::min(nx, t.x0 + tx * t.vw + t.vw + t.halo);
   };
   auto touches = [&](int tx, int r0, int r1) {
-    if (fdtd2d::tile_touches_cpml(nx, ny, P, t.vw, t.halo, T, tx, r0, r1)) return true;
+    if (fdtd2d::tile_touches_cpml(nx, ny, P, t.x0, t.vw, t.halo, T, tx, r0, r1)) return true;
     if (!m) return false;
     int jl0, jl1, cs, ce;
     window(tx, r0, r1, jl0, jl1, cs, ce);
@@ -213,14 +223,24 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
         if (touches(tx, r0, r1)) {
           if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, r0, 0));
           run0 = -1;
-          const int c1 = std::min(nx, tx * t.vw + t.vw);
-          for (int c0 = tx * t.vw; c0 < c1; c0 += t.vw_e)
+          const int c1 = std::min(nx, t.x0 + tx * t.vw + t.vw);
+          for (int c0 = t.x0 + tx * t.vw; c0 < c1; c0 += t.vw_e)
             lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, c1 - c0)), r0, r1, 0));
         } else if (run0 < 0) {
           run0 = r0;
         }
       }
       if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, jb1, 0));
+    }
+  // the margins left and right of the interior tiles (C > 1): CPML-class
+  // pieces on sub-bands of hc rows
+  if (C > 1)
+    for (int r0 = 0; r0 < ny; r0 += t.hc) {
+      const int r1 = std::min(ny, r0 + t.hc);
+      for (int c0 = 0; c0 < std::min(t.x0, nx); c0 += t.vw_e)
+        lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, std::min(t.x0, nx) - c0)), r0, r1, 0));
+      for (int c0 = std::max(xr, t.x0); c0 < nx; c0 += t.vw_e)
+        lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, nx - c0)), r0, r1, 0));
     }
   // Order by column group of 4 tiles, then first row, then tile: the 4 waves
   // of a work-group get neighbouring tiles and consecutive work-groups go down
@@ -238,7 +258,7 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
     });
     for (int4 &w : lists[c]) {
       w.w = c == 0 ? needs_jz(w) : 1;
-      const long long c0 = c == 0 ? static_cast<long long>(w.x) * t.vw : edge_entry_c0(w.x);
+      const long long c0 = c == 0 ? t.x0 + static_cast<long long>(w.x) * t.vw : edge_entry_c0(w.x);
       const long long c1 = c == 0 ? std::min<long long>(nx, c0 + t.vw) : c0 + edge_entry_w(w.x);
       t.cells[c] += static_cast<double>(std::max<long long>(0, c1 - c0)) * (w.z - w.y);
     }
@@ -267,6 +287,7 @@ int set_tiling(fdtd2d_plan *pl) {
   a.vw = t.vw;
   a.halo_e = t.halo_e;
   a.vw_e = t.vw_e;
+  a.x0 = t.x0;
   a.trash_slots = std::min(kTrashSlots, kTrashRows * pl->L.pitch / (64 * pl->C));
   a.ntx = t.ntx;
   a.H = t.H;
@@ -385,7 +406,7 @@ int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, i
   for (int c = 0; c < 2; ++c)
     for (int q = 0; q < t.n_tiles[c]; ++q, ++k) {
       const int4 &w = t.lists[c][q];
-      const int c0 = c == 0 ? w.x * t.vw : edge_entry_c0(w.x);
+      const int c0 = c == 0 ? t.x0 + w.x * t.vw : edge_entry_c0(w.x);
       const int c1 = c == 0 ? std::min(nx, c0 + t.vw) : c0 + edge_entry_w(w.x);
       int *e = entries + 5 * k;
       e[0] = c0;

@@ -60,6 +60,11 @@ inline int edge_entry_w(int x) { return x >> kEdgeColBits; }
 struct Tiling {
   int T = 0, halo = 0, vw = 0, ntx = 0, H = 0, nty = 0, hc = 0;
   int halo_e = 0, vw_e = 0;  // CPML-class waves: 1 column per lane
+  // Interior tile tx stores columns from x0 + tx*vw. With more than one
+  // column per lane the interior tiles start after a left margin of CPML-class
+  // pieces wide enough for the x-strip and end before a right margin of
+  // width >= x0; with one column per lane x0 = 0 and the tiles cover all.
+  int x0 = 0;
   std::vector<int4> lists[2];  // padded to whole work-groups
   int n_tiles[2] = {0, 0};
   int n_nojz = 0;  // interior tiles that skip the Jz update

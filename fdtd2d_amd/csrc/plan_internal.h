@@ -82,6 +82,7 @@ struct StepArgs {
   int vw;        // stored columns per tile = 64*C - 2*halo
   int halo;      // recomputed columns on each side, >= steps per launch
   int vw_e, halo_e;  // the same for CPML-class waves (1 column per lane)
+  int x0;            // first column of interior tile 0
   int H;         // rows per band
   int ntx, nty;  // tiles across / down
   // Tile lists, one entry per wave = {tile, first row, end row, flags}; wave
@@ -153,15 +154,15 @@ hipError_t launch_pool_tick(PoolQueue *q, int nk, int dg, hipStream_t stream);
 hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,
                            hipStream_t stream);
 
-// True when tile tx (vw stored columns, halo recomputed columns each side)
-// over output rows [jb0, jb1) loads a cell of a CPML strip or of the outer
-// ring (with no CPML the ring counts as a strip of width 1): such a tile
-// takes the CPML-class kernel.
-inline bool tile_touches_cpml(int nx, int ny, int P, int vw, int halo, int T, int tx, int jb0, int jb1) {
+// True when tile tx (columns from x0 + tx*vw, vw stored, halo recomputed
+// columns each side) over output rows [jb0, jb1) loads a cell of a CPML
+// strip or of the outer ring (with no CPML the ring counts as a strip of
+// width 1): such a tile takes the CPML-class kernel.
+inline bool tile_touches_cpml(int nx, int ny, int P, int x0, int vw, int halo, int T, int tx, int jb0, int jb1) {
   const int Pe = P > 0 ? P : 1;
   const int jl0 = jb0 - T > 0 ? jb0 - T : 0, jl1 = jb1 + T < ny ? jb1 + T : ny;
   if (jl0 <= Pe || jl1 >= ny - Pe - 1) return true;
-  const int cs = tx * vw - halo, ce = cs + vw + 2 * halo;
+  const int cs = x0 + tx * vw - halo, ce = cs + vw + 2 * halo;
   return cs <= Pe || ce >= nx - Pe - 1;
 }
 

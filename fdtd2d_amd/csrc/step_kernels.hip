@@ -565,7 +565,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
     own1 = min(nx, own0 + (x >> kEdgeColBits));
     cs = own0 - a.halo_e;  // halo_e >= T
   } else {
-    own0 = x * a.vw;  // vw = 64*C - 2*halo, halo >= T and a multiple of C
+    own0 = a.x0 + x * a.vw;  // vw = 64*C - 2*halo, halo >= T and a multiple of C
     own1 = min(nx, own0 + a.vw);
     cs = own0 - a.halo;
   }

@@ -313,7 +313,9 @@ int fdtd2d_write_psi(fdtd2d_plan *plan, int which, const float *in, size_t count
    column, end column, first row, end row, class} of the cells wave k stores,
    for k < *count; class 0 interior kernel (cols_per_lane columns per lane),
    1 CPML-class kernel (always one column per lane: a CPML-class tile of the
-   interior geometry is split into pieces of 64 - 2*tblock columns).
+   interior geometry is split into pieces of 64 - 2*tblock columns; with more
+   than one column per lane the interior tiles start after, and end before,
+   margins of such pieces as wide as the x-strip and the halo need).
    geometry (4 ints, may be null) = {vw, halo columns, rows per band, rows per
    CPML sub-band} of the interior tiles. entries may be null to query *count.
    tblock 0 = the default. No reference counterpart: the reference launches

@@ -26,7 +26,7 @@ def check_partition(nx, ny, P, T, C, band, band_cpml):
     for c0, c1, r0, r1, cls in ent:
         assert 0 <= r0 < r1 <= ny and 0 <= c0 < c1 <= nx
         if cls == 0:
-            assert c0 % vw == 0 and c1 == min(nx, c0 + vw)
+            assert c1 == min(nx, c0 + vw)
         else:
             assert c1 - c0 <= vw_e
         cover[r0:r1, c0:c1] += 1
@@ -72,6 +72,15 @@ def test_partition_8192_default():
     ntx = -(-8192 // g["vw"])
     mid = {c0 // g["vw"] for c0, _, r0, _, c in ent if c == 1 and 200 < r0 < 7900}
     assert mid == {0, ntx - 1}
+
+
+def test_partition_8192_two_columns():
+    # two columns per lane: the x-strips go to narrow one-column margins, so
+    # the CPML-class share stays near the one-column tiling's
+    ent, g, cells = check_partition(8192, 8192, 16, 8, 2, 96, 32)
+    assert cells[1] / (8192 * 8192) < 0.03
+    mid = sorted({(c0, c1) for c0, c1, r0, _, c in ent if c == 1 and 200 < r0 < 7900})
+    assert all(c1 <= 96 or c0 >= 8192 - 200 for c0, c1 in mid)
 
 
 def test_event_cells_take_cpml_class():
