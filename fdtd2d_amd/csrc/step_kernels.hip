@@ -693,8 +693,10 @@ typedef const StepArgs __attribute__((address_space(4))) *cargs_ptr;
 // ones first (one kernel for a whole launch: every wave takes CPML-class work
 // while there is any, so the longer CPML-class tiles never run at the tail
 // with the rest of the device idle).
+// The CPML-class form (MODE 1) needs more than the 128 VGPRs of 4 waves per
+// SIMD at T = 8 and runs at 3 rather than spill.
 template <int T, int C, int D, int MED, int MODE>
-__global__ void __launch_bounds__(64 * kWavesPerWG, 4)
+__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 1 ? 3 : 4)
     pool_kernel(const StepArgs *__restrict__ descs, PoolQueue *q, int k, int tp) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   __shared__ int s_pos, s_cls;
