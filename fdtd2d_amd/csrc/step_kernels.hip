@@ -633,11 +633,6 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
   // compiler allocates for (occupancy 2); there one loop checks every level.
   const int kwarm = C > 1 ? jl0 : jb0 + T - 1, kend = jb1 + T;
   int k = jl0;
-  // Each loop is entered with no load in flight: the wait counts at a loop
-  // header merge the entry path with the back edge, and a prefetch still in
-  // flight on entry would make every iteration wait for all memory
-  // operations, the previous iteration's stores included.
-  __builtin_amdgcn_s_waitcnt(0);
   for (; k < kwarm; k += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
@@ -649,7 +644,6 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
                                           trash_off, in, st);
     }
   }
-  __builtin_amdgcn_s_waitcnt(0);
   for (; k < kend; k += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
