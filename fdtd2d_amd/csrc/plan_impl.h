@@ -103,6 +103,7 @@ struct fdtd2d_plan {
   int band = 0;  // 0: automatic
   int D = 2;     // rows of prefetch per wave (interior kernel); the row loop is unrolled by D
   int band_cpml = 0;               // rows per CPML-class work-group (0: automatic)
+  int jz_first = 1;  // interior work-groups with a Jz update first in each XCD's range
   double cells_class[2] = {0, 0};  // grid cells owned by interior / CPML-class tiles
   int hc = 0;                      // rows per CPML-class sub-band in use
   int n_nojz = 0;                  // interior tiles skipping the Jz update

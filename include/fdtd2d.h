@@ -288,11 +288,13 @@ int fdtd2d_plan_set_tblock(fdtd2d_plan *plan, int steps_per_launch);
    tile, >= tblock; 0 = automatic), "band_cpml" (rows per tile of the tiles
    that touch the CPML, 0 = automatic), "cols_per_lane" (1, 2 or 4), "prefetch"
    (rows loaded ahead per wave: 1, 2 or 4 with 1 column per lane, 1 or 2 with
-   2, 1 with 4). Results are identical for every choice. */
+   2, 1 with 4), "jz_first" (1, the default: interior work-groups that update
+   Jz are launched first on each XCD; 0: column order only). Results are
+   identical for every choice. */
 int fdtd2d_plan_set_option(fdtd2d_plan *plan, const char *name, double value);
 /* Plan facts by name: "pitch", "tblock", "band", "band_cpml", "ntx", "nty",
    "workgroups", "workgroups_cpml", "tile_cols", "bytes", "steps",
-   "cols_per_lane", "prefetch", "cells_interior", "cells_cpml". */
+   "cols_per_lane", "prefetch", "jz_first", "cells_interior", "cells_cpml". */
 int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
 /* Advance nsteps steps (asynchronous on the plan's stream). */
 int fdtd2d_run(fdtd2d_plan *plan, int nsteps);

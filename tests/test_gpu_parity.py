@@ -84,6 +84,7 @@ def test_random_small_problems(gpu_device, seed):
         plan.set_option("band", band)
         plan.set_option("prefetch", depth)
         plan.set_option("band_cpml", int(rng.choice([0, 1, 5, 9])))
+        plan.set_option("jz_first", int(rng.integers(0, 2)))
         plan.run(p.nsteps)
         plan.sync()
         g = all_state(plan, p.npml)
