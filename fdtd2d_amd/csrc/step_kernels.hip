@@ -117,6 +117,7 @@ struct Cols {
   float hyx_b[EDGE ? C : 1], hyx_c[EDGE ? C : 1], ezx_b[EDGE ? C : 1], ezx_c[EDGE ? C : 1];
   int hyx_s[EDGE ? C : 1], ezx_s[EDGE ? C : 1];
   int probe[EDGE ? C : 1];  // RC capture probe of the column, -1 none
+  bool wave_x;              // some lane of the wave has an x-strip column
 };
 
 // Per-row scalars of the row a level works on (uniform across the wave):
@@ -143,19 +144,23 @@ __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
     return q;
   }
   const bool in = r >= 0 && r < a.ny;
-  const long long ri = static_cast<long long>(kRowRec) * (in ? r : 0);
-  q.den_hy = __int_as_float(cld_i(a.rowinfo, ri));
-  q.den_ey = __int_as_float(cld_i(a.rowinfo, ri + 1));
-  const int e0 = cld_i(a.rowinfo, ri + 2), e1 = cld_i(a.rowinfo, ri + 3);
-  q.e0 = in && e0 >= 0 ? e0 : 0;
-  q.e1 = in && e0 >= 0 ? e1 : 0;
-  const int hs = cld_i(a.rowinfo, ri + 4), es = cld_i(a.rowinfo, ri + 5);
-  q.hxy_s = in ? hs : -1;
-  q.ezy_s = in ? es : -1;
-  q.hxy_b = __int_as_float(cld_i(a.rowinfo, ri + 6));
-  q.hxy_c = __int_as_float(cld_i(a.rowinfo, ri + 7));
-  q.ezy_b = __int_as_float(cld_i(a.rowinfo, ri + 8));
-  q.ezy_c = __int_as_float(cld_i(a.rowinfo, ri + 9));
+  // the whole record in three 16-byte scalar loads issued together (loads of
+  // single fields would be sunk into the branches that use them, each then
+  // waiting for its own load)
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  typedef const v4i __attribute__((address_space(4))) *cv4i_ptr;
+  const cv4i_ptr rec = (cv4i_ptr)a.rowinfo + static_cast<long long>(kRowRec / 4) * (in ? r : 0);
+  const v4i w0 = rec[0], w1 = rec[1], w2 = rec[2];
+  q.den_hy = __int_as_float(w0.x);
+  q.den_ey = __int_as_float(w0.y);
+  q.e0 = in && w0.z >= 0 ? w0.z : 0;
+  q.e1 = in && w0.z >= 0 ? w0.w : 0;
+  q.hxy_s = in ? w1.x : -1;
+  q.ezy_s = in ? w1.y : -1;
+  q.hxy_b = __int_as_float(w1.z);
+  q.hxy_c = __int_as_float(w1.w);
+  q.ezy_b = __int_as_float(w2.x);
+  q.ezy_c = __int_as_float(w2.y);
   return q;
 }
 
@@ -193,15 +198,19 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
       o.phyx[c] = p.phyx[c];
       hx = updH ? hx : p.hx[c];
       hy = updH ? hy : p.hy[c];
-      if (updH && rs.hxy_s >= 0) {
+      // psi strips: the branches test wave-uniform conditions (the row's
+      // strip slot, or some lane of the wave in an x strip) and the lanes
+      // select, so no branch splits the wave
+      if (rs.hxy_s >= 0) {
         const float ps = rs.hxy_b * p.phxy[c] + div_by(rs.hxy_c * dEy, a.rdy);
-        o.phxy[c] = ps;
-        hx = hx + a.db * ps;
+        o.phxy[c] = updH ? ps : p.phxy[c];
+        hx = updH ? hx + a.db * ps : hx;
       }
-      if (updH && cl.hyx_s[c] >= 0) {
+      if (cl.wave_x) {
+        const bool u = updH && cl.hyx_s[c] >= 0;
         const float ps = cl.hyx_b[c] * p.phyx[c] + div_by(cl.hyx_c[c] * dEx, a.rdx);
-        o.phyx[c] = ps;
-        hy = hy + a.db * ps;
+        o.phyx[c] = u ? ps : p.phyx[c];
+        hy = u ? hy + a.db * ps : hy;
       }
     }
     o.hx[c] = hx;
@@ -252,15 +261,16 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
       ez = updE ? ez : p.ez[c];
       o.pezx[c] = p.pezx[c];
       o.pezy[c] = p.pezy[c];
-      if (updE && cl.ezx_s[c] >= 0) {
+      if (cl.wave_x) {
+        const bool u = updE && cl.ezx_s[c] >= 0;
         const float ps = cl.ezx_b[c] * p.pezx[c] + div_by(cl.ezx_c[c] * dHy, a.rdx);
-        o.pezx[c] = ps;
-        ez = ez + cb * ps;
+        o.pezx[c] = u ? ps : p.pezx[c];
+        ez = u ? ez + cb * ps : ez;
       }
-      if (updE && rs.ezy_s >= 0) {
+      if (rs.ezy_s >= 0) {
         const float ps = rs.ezy_b * p.pezy[c] + div_by(rs.ezy_c * dHx, a.rdy);
-        o.pezy[c] = ps;
-        ez = ez + cb * ps;
+        o.pezy[c] = updE ? ps : p.pezy[c];
+        ez = updE ? ez + cb * ps : ez;
       }
     }
     o.jz[c] = jz;
@@ -589,6 +599,13 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       cl.ezx_c[c] = in ? gp(a.ezx_c)[col] : 0.f;
       cl.probe[c] = in ? gp(a.probe_col)[col] : -1;
     }
+  }
+  cl.wave_x = false;
+  if constexpr (EDGE) {
+    bool x = false;
+#pragma unroll
+    for (int c = 0; c < C; ++c) x = x || cl.hyx_s[c] >= 0 || cl.ezx_s[c] >= 0;
+    cl.wave_x = __builtin_amdgcn_ballot_w64(x) != 0;
   }
   Streams sm, dst;
   sm.ez = (const gchar *)a.ez[src];
