@@ -166,7 +166,7 @@ __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
 
 // One level: level t-1's row r (p), Ez^{t-1}(r+1) (ez_s) and Hx^{t-1/2}(r-1)
 // (hx_n) -> level t's row r (o).
-template <int C, bool EDGE, int MED, bool JZ, bool EV>
+template <int C, bool EDGE, int MED, bool JZ, bool EV, bool XT>
 __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl,
                                         const RowS<EDGE> &rs, const Level<C, EDGE> &p, const float (&ez_s)[C],
                                         const float (&hx_n)[C], int r, int step, int own0, int own1,
@@ -301,7 +301,7 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
       }
     }
   }
-  if constexpr (EDGE) {
+  if constexpr (EDGE && XT) {
     // RC capture of probe columns (after the sources, as the reference's
     // Dev_RCcalc reads Ez after the detectors of the step)
     if (a.nprobe > 0 && own_row) {
@@ -485,7 +485,7 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
 // level and all later ones are skipped. A skipped level leaves its pipeline
 // register as it was; the first needed row of a level reads only its Hx
 // from that register's rows, and only for an Ez nothing needs.
-template <int T, int C, bool EDGE, int MED, bool JZ, bool EV, bool WARM>
+template <int T, int C, bool EDGE, int MED, bool JZ, bool EV, bool XT, bool WARM>
 __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl, int n0,
                                        int k, int jb0, int jb1, int own0, int own1, Level<C, EDGE> prev,
                                        Level<C, EDGE> (&st)[T + 1]) {
@@ -501,7 +501,7 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
     Level<C, EDGE> out;
     const bool own_row = r >= jb0 && r < jb1;
     const RowS<EDGE> rs = row_scalars<EDGE>(a, r);
-    advance<C, EDGE, MED, JZ, EV>(a, lut_s, cl, rs, prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
+    advance<C, EDGE, MED, JZ, EV, XT>(a, lut_s, cl, rs, prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
     prev = st[t];
     st[t] = out;
 #pragma unroll
@@ -511,7 +511,7 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
 
 // One row iteration: level 0 takes row k, level t computes row k-t, and
 // row k-T is stored.
-template <int T, int C, bool EDGE, int MED, bool JZ, bool WARM>
+template <int T, int C, bool EDGE, int MED, bool JZ, bool XT, bool WARM>
 __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, const float4 *lut_s,
                                          const Cols<C, EDGE> &cl, int src, int n0, int k, int jb0, int jb1,
                                          int own0, int own1, bool lane_owns, unsigned trash_off,
@@ -519,7 +519,7 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
   Level<C, EDGE> prev = st[0];
   st[0] = in;
   // ---- levels 1..T, each one row behind the previous
-  levels<T, C, EDGE, MED, JZ, EDGE, WARM>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st);
+  levels<T, C, EDGE, MED, JZ, EDGE, XT, WARM>(a, lut_s, cl, n0, k, jb0, jb1, own0, own1, prev, st);
   // ---- store step n0+T for owned cells of row k-T
   const int r = k - T;
   if constexpr (!EDGE) {
@@ -558,12 +558,20 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
   }
 }
 
+// CPML-class tiles compile their row loop twice: with the RC probe capture
+// and the processing line capture (XT), and without, for the launches of a
+// problem that has neither (their per-level tests are about a tenth of the
+// CPML-class kernel's instructions).
+__device__ __forceinline__ bool extras_on(const StepArgs &a) {
+  return a.nprobe > 0 || (a.line_i1 > a.line_i0 && a.line_end > a.line_begin);
+}
+
 // D = rows fetched ahead of the one being consumed (memory-level
 // parallelism per wave); the row loop is unrolled by D so the prefetch ring
 // is indexed statically.
 // x: the interior entry's tile index (columns [x*vw, x*vw+vw)), or the
 // CPML-class entry's packed first column and width (one column per lane).
-template <int T, int C, int D, bool EDGE, int MED, bool JZ>
+template <int T, int C, int D, bool EDGE, int MED, bool JZ, bool XT = false>
 __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s, int src, int n0,
                                          int x, int jb0, int jb1) {
   static_assert(!EDGE || C == 1, "CPML-class tiles run one column per lane");
@@ -657,7 +665,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       const int kn = k + u + D;
       const int kk = EDGE ? kn : min(kn, jl1 - 1);
       load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
-      row_step<T, C, EDGE, MED, JZ, true>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
+      row_step<T, C, EDGE, MED, JZ, XT, true>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
                                           trash_off, in, st);
     }
   }
@@ -668,7 +676,7 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
       const int kn = k + u + D;
       const int kk = EDGE ? kn : min(kn, jl1 - 1);
       load_level0<C, EDGE, MED, JZ>(a, sm, cl, src, kk, kn < jl1, lane_in, pre[u]);
-      row_step<T, C, EDGE, MED, JZ, (C > 1)>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
+      row_step<T, C, EDGE, MED, JZ, XT, (C > 1)>(a, dst, lut_s, cl, src, n0, k + u, jb0, jb1, own0, own1, lane_owns,
                                              trash_off, in, st);
     }
   }
@@ -693,10 +701,16 @@ __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a
   const int tx = cld_i(a.tile_list[EDGE ? 1 : 0], e), r0 = cld_i(a.tile_list[EDGE ? 1 : 0], e + 1),
             r1 = cld_i(a.tile_list[EDGE ? 1 : 0], e + 2), flags = cld_i(a.tile_list[EDGE ? 1 : 0], e + 3);
   if (r0 >= r1) return;  // padding entry
-  if (EDGE || (flags & 1))
-    run_tile<T, C, D, EDGE, MED, true>(a, lut_s, src, n0, tx, r0, r1);
-  else
-    run_tile<T, C, D, EDGE, MED, false>(a, lut_s, src, n0, tx, r0, r1);
+  if constexpr (EDGE) {
+    if (extras_on(a))
+      run_tile<T, C, D, true, MED, true, true>(a, lut_s, src, n0, tx, r0, r1);
+    else
+      run_tile<T, C, D, true, MED, true, false>(a, lut_s, src, n0, tx, r0, r1);
+  } else if (flags & 1) {
+    run_tile<T, C, D, false, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+  } else {
+    run_tile<T, C, D, false, MED, false>(a, lut_s, src, n0, tx, r0, r1);
+  }
 }
 
 // Persistent form for pools of independent shots: each work-group takes
@@ -766,12 +780,20 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 1 ? 3 : 4)
     const int src = a.src_base ^ (dg & 1);
     if (r0 < r1) {
       if constexpr (MODE == 1) {
-        run_tile<T, C, 1, true, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+        if (extras_on(a))
+          run_tile<T, C, 1, true, MED, true, true>(a, lut_s, src, n0, tx, r0, r1);
+        else
+          run_tile<T, C, 1, true, MED, true, false>(a, lut_s, src, n0, tx, r0, r1);
       } else {
         bool edge_tile = false;
         if constexpr (MODE == 2) edge_tile = cls == 1;
         if constexpr (MODE == 2) {
-          if (edge_tile) run_tile<T, C, 1, true, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+          if (edge_tile) {
+            if (extras_on(a))
+              run_tile<T, C, 1, true, MED, true, true>(a, lut_s, src, n0, tx, r0, r1);
+            else
+              run_tile<T, C, 1, true, MED, true, false>(a, lut_s, src, n0, tx, r0, r1);
+          }
         }
         if (!edge_tile) {
           if (w & 1)
