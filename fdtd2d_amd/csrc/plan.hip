@@ -86,7 +86,7 @@ Layout make_layout(const fdtd2d_problem *p, size_t n_events) {
   L.off_lut = take(256 * 16);
   for (int f = 0; f < 8; ++f) L.off_col[f] = take(static_cast<size_t>(L.pitch) * 4);
   for (int f = 0; f < 8; ++f) L.off_row[f] = take(static_cast<size_t>(L.ny) * 4);
-  L.off_rowinfo = take(static_cast<size_t>(L.ny) * fdtd2d::kRowRec * 4);
+  L.off_rowinfo = take(static_cast<size_t>(L.ny + 2 * fdtd2d::kMaxT) * fdtd2d::kRowRec * 4);
   L.off_evoff = take(static_cast<size_t>(L.ny + 1) * 4);
   L.off_events = take(n_events * sizeof(fdtd2d::Event));
   L.off_src = take(static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4);
@@ -623,7 +623,15 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     if ((rc = upload(pl, L.off_row[6], hxy_s.data(), ny * 4))) return fail(rc);
     if ((rc = upload(pl, L.off_row[7], ezy_s.data(), ny * 4))) return fail(rc);
     // the per-row record the CPML-class kernel reads at each level
-    std::vector<int32_t> ri(static_cast<size_t>(ny) * fdtd2d::kRowRec, 0);
+    // (kMaxT records on each side for the rows the levels reach outside the
+    // grid: no events, no strips)
+    const int pad = fdtd2d::kMaxT;
+    std::vector<int32_t> ri(static_cast<size_t>(ny + 2 * pad) * fdtd2d::kRowRec, 0);
+    for (int j = -pad; j < ny + pad; ++j) {
+      int32_t *q = &ri[static_cast<size_t>(j + pad) * fdtd2d::kRowRec];
+      q[2] = -1;
+      q[4] = q[5] = -1;
+    }
     auto bits = [](float v) {
       int32_t b;
       std::memcpy(&b, &v, 4);
@@ -631,7 +639,7 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     };
     for (int j = 0; j < ny; ++j) {
       const int jc = std::min(j, ny - 2);
-      int32_t *q = &ri[static_cast<size_t>(j) * fdtd2d::kRowRec];
+      int32_t *q = &ri[static_cast<size_t>(j + pad) * fdtd2d::kRowRec];
       q[0] = bits(p->den_hy[jc]);
       q[1] = bits(p->den_ey[jc]);
       q[2] = ev_off[j] < ev_off[j + 1] ? ev_off[j] : -1;
@@ -773,7 +781,7 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   a.hxy_slot = reinterpret_cast<const int32_t *>(pl->base + L.off_row[6]);
   a.ezy_slot = reinterpret_cast<const int32_t *>(pl->base + L.off_row[7]);
   a.ev_off = reinterpret_cast<const int32_t *>(pl->base + L.off_evoff);
-  a.rowinfo = reinterpret_cast<const int *>(pl->base + L.off_rowinfo);
+  a.rowinfo = reinterpret_cast<const int *>(pl->base + L.off_rowinfo) + fdtd2d::kMaxT * fdtd2d::kRowRec;
   a.events = reinterpret_cast<const fdtd2d::Event *>(pl->base + L.off_events);
   a.src_table = reinterpret_cast<const float *>(pl->base + L.off_src);
   a.nsrc = p->nsrc;

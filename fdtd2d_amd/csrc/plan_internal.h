@@ -51,7 +51,8 @@ struct StepArgs {
   const int32_t *hxy_slot, *ezy_slot;          // [ny]
   // per row, kRowRec ints: {den_hy bits, den_ey bits, first event or -1 when
   // the row has none, end event, psi_hxy slot, psi_ezy slot (-1 outside the
-  // y strips), hxy b, hxy c, ezy b, ezy c (bits), 0, 0}, [ny]
+  // y strips), hxy b, hxy c, ezy b, ezy c (bits), 0, 0}, for rows
+  // -kMaxT .. ny+kMaxT-1 (the rows outside the grid: no events, no strips)
   const int *rowinfo;
   // events by row
   const int32_t *ev_off;  // [ny+1]

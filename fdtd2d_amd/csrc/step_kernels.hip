@@ -143,20 +143,20 @@ __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
     q.e0 = q.e1 = 0;
     return q;
   }
-  const bool in = r >= 0 && r < a.ny;
   // the whole record in three 16-byte scalar loads issued together (loads of
   // single fields would be sunk into the branches that use them, each then
-  // waiting for its own load)
+  // waiting for its own load); the records cover rows -kMaxT .. ny+kMaxT-1,
+  // those outside the grid with no events and no strips
   typedef int v4i __attribute__((ext_vector_type(4)));
   typedef const v4i __attribute__((address_space(4))) *cv4i_ptr;
-  const cv4i_ptr rec = (cv4i_ptr)a.rowinfo + static_cast<long long>(kRowRec / 4) * (in ? r : 0);
+  const cv4i_ptr rec = (cv4i_ptr)a.rowinfo + (kRowRec / 4) * r;
   const v4i w0 = rec[0], w1 = rec[1], w2 = rec[2];
   q.den_hy = __int_as_float(w0.x);
   q.den_ey = __int_as_float(w0.y);
-  q.e0 = in && w0.z >= 0 ? w0.z : 0;
-  q.e1 = in && w0.z >= 0 ? w0.w : 0;
-  q.hxy_s = in ? w1.x : -1;
-  q.ezy_s = in ? w1.y : -1;
+  q.e0 = w0.z >= 0 ? w0.z : 0;
+  q.e1 = w0.z >= 0 ? w0.w : 0;
+  q.hxy_s = w1.x;
+  q.ezy_s = w1.y;
   q.hxy_b = __int_as_float(w1.z);
   q.hxy_c = __int_as_float(w1.w);
   q.ezy_b = __int_as_float(w2.x);
