@@ -38,6 +38,10 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
+# launch shape of the serving pool (pool options); the serving streams mix
+# shot sizes like the batch mix, whose autotuned shape it is
+SERVE_SHAPE = {"fused": 0, "tblock": 8, "band": 256, "wg_per_cu": 3, "band_cpml": 32, "wg_per_cu_cpml": 3,
+               "prefetch": 1}
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
@@ -240,7 +244,9 @@ def bench_serve(args) -> int:
     nshots = sum(len(r.shots) for r in reqs)
     log(f"{args.mode}: {len(reqs)} requests ({nshots} shots) at {rate} req/s over {args.duration}s; "
         f"drawn in {time.time() - t0:.1f}s")
-    srv = Server(0, budget_bytes=budget, tblock=8, band=128, wg_per_cu=3, wg_per_cu_cpml=2, prefetch=1)
+    shape = json.loads(args.pool_shape)
+    log(f"pool shape {shape}")
+    srv = Server(0, budget_bytes=budget, **shape)
     done, st = srv.run(reqs, max_seconds=args.duration + args.drain, log=log)
     srv.close()
     lat = latency_summary(done, classes)
@@ -290,6 +296,8 @@ def main() -> int:
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--pool-shape", default=json.dumps(SERVE_SHAPE),
+                    help="launch shape of the serving pool, as JSON pool options")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)

@@ -18,14 +18,14 @@ from concurrent.futures import ThreadPoolExecutor
 from .problem import Pool, Setup
 
 # coordinate-descent axes, in the order they are tuned; the first value of
-# each is the starting point
+# each is the starting point (the shape that won on uneven shot mixes)
 TUNE_AXES = [
     ("tblock", [8, 6, 7]),
-    ("band", [128, 64, 256]),
-    ("wg_per_cu", [4, 3]),
+    ("band", [256, 128, 64]),
+    ("wg_per_cu", [3, 4]),
     ("band_cpml", [32, 16, 64]),
-    ("wg_per_cu_cpml", [1, 2, 3]),
-    ("prefetch", [2, 1]),
+    ("wg_per_cu_cpml", [3, 1, 2]),
+    ("prefetch", [1, 2]),
 ]
 
 
@@ -68,18 +68,33 @@ def _descend(pool: Pool, start: dict, axes, launches_per_trial: int, trials: lis
     return best, best_rate
 
 
-def autotune(pool: Pool, launches_per_trial: int = 4, axes=TUNE_AXES, log=None) -> dict:
+def autotune(pool: Pool, launches_per_trial: int = 4, axes=TUNE_AXES, log=None, margin: float = 0.95) -> dict:
     """Picks the launch shape for the shots now in the pool; returns it with
-    the score of every trial. One coordinate descent per kernel arrangement
-    (one fused kernel per launch, or one kernel per tile class), the better
-    one kept. Stops early when the pool runs out of work."""
+    the score of every trial. A coordinate descent with one kernel per tile
+    class, then one trial of the fused kernel at the shape found, descended
+    from only when it scores within `margin` of the best (every trial runs
+    the real shots, so a descent at a losing arrangement costs throughput).
+    Stops early when the pool runs out of work."""
     trials = []
     found = []
-    for fused in (1, 0):
-        if pool.info("running") == 0:
-            break
-        start = {"fused": fused, **{name: values[0] for name, values in axes}}
+    if pool.info("running") > 0:
+        start = {"fused": 0, **{name: values[0] for name, values in axes}}
+        for name, value in start.items():
+            pool.set_option(name, value)
+        # the first launches after the shots are admitted run slower (graph
+        # capture, cold caches): run them before scoring the starting shape
+        _trial(pool, launches_per_trial)
         found.append(_descend(pool, start, axes, launches_per_trial, trials, log))
+    if found and pool.info("running") > 0:
+        shape = {**found[0][0], "fused": 1}
+        for name, value in shape.items():
+            pool.set_option(name, value)
+        rate = _trial(pool, launches_per_trial)
+        trials.append((dict(shape), rate))
+        if log:
+            log(f"autotune fused=1 at that shape: {rate / 1e9:.1f} Gcell/s (best {found[0][1] / 1e9:.1f})")
+        if rate >= margin * found[0][1]:
+            found.append(_descend(pool, shape, axes, launches_per_trial, trials, log))
     if not found:
         return {"shape": {}, "rate": 0.0, "trials": trials}
     best, best_rate = max(found, key=lambda f: f[1])
