@@ -128,9 +128,10 @@ struct fdtd2d_pool {
   hipStream_t stream = nullptr, aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // launch shape
-  int T = 8, band = 0, band_cpml = 0, D = 2;
-  int wg_per_cu = 4, wg_per_cu_cpml = 1;
-  int fused = 1;  // one kernel per launch, CPML-class work first (0: two kernels on two streams)
+  // (defaults: the shape the autotuner picks for mixed shot sizes)
+  int T = 8, band = 256, band_cpml = 32, D = 1;
+  int wg_per_cu = 3, wg_per_cu_cpml = 3;
+  int fused = 0;  // 0: one kernel per tile class on two streams; 1: one kernel, CPML-class work first
   int chunk = 8;  // launches per captured graph; 0 = no graphs
   int medium_kind = -1;
   // memory

@@ -357,12 +357,14 @@ int fdtd2d_plan_kernel_ms(fdtd2d_plan *plan, double *avg_ms, int *count);
  * `graph_chunk` launches are replayed from a captured hipGraph when no shot
  * ends inside them). Shots can be added and removed between steps.
  *
- * Options (fdtd2d_pool_set_option): "tblock" (1..8, default 8), "band",
- * "band_cpml", "prefetch" (1 or 2), "fused" (1, the default: one persistent
- * kernel per launch that takes the CPML-class tiles first, then the interior
- * ones; 0: one kernel per class on two streams), "wg_per_cu" /
+ * Options (fdtd2d_pool_set_option): "tblock" (1..8, default 8), "band"
+ * (default 256, 0 = per shot), "band_cpml" (default 32, 0 = per shot),
+ * "prefetch" (1, the default, or 2), "fused" (0, the default: one persistent
+ * kernel per tile class on two streams; 1: one kernel per launch that takes
+ * the CPML-class tiles first, then the interior ones), "wg_per_cu" /
  * "wg_per_cu_cpml" (persistent work-groups per CU of the interior or fused
- * kernel / of the CPML-class kernel), "graph_chunk" (0..16, 0 = no graphs),
+ * kernel / of the CPML-class kernel, default 3 / 3), "graph_chunk" (0..16,
+ * 0 = no graphs),
  * "budget_bytes" (before any shot: carve shots from one arena of that size
  * and refuse shots that do not fit, FDTD2D_ENOMEM).
  * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",

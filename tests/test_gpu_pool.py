@@ -61,13 +61,15 @@ def _problems(seed, n, table=False, sizes=(80, 330), nsteps=None, steps=(9, 45))
     return out
 
 
-@pytest.mark.parametrize("table,chunk,tblock", [(False, 4, 8), (True, 4, 8), (False, 0, 5), (True, 2, 3)])
-def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock):
+@pytest.mark.parametrize("table,chunk,tblock,fused,depth", [(False, 4, 8, 1, 2), (True, 4, 8, 0, 1),
+                                                             (False, 0, 5, 0, 2), (True, 2, 3, 1, 1)])
+def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock, fused, depth):
     """Uneven shots (sizes, CPML, steps not multiples of T) in one pool."""
     probs = _problems(900 + 10 * tblock + chunk, 7, table=table, steps=(45, 90))
-    with F.Pool(gpu_device, tblock=tblock, graph_chunk=chunk) as pool:
+    with F.Pool(gpu_device, tblock=tblock, graph_chunk=chunk, fused=fused, prefetch=depth, band=0,
+                band_cpml=0) as pool:
         ids = [pool.add(p) for p in probs]
-        print(f"table={table} chunk={chunk} T={tblock}: shots "
+        print(f"table={table} chunk={chunk} T={tblock} fused={fused} D={depth}: shots "
               f"{[(p.nx, p.ny, p.npml, p.nsteps) for p in probs]}", flush=True)
         _drain(pool)
         print({k: pool.info(k) for k in ("launches", "graph_replays", "eager_launches", "tail_launches")})
