@@ -12,7 +12,8 @@ Prints ONE JSON line (rank 0). Besides the metric it carries
                 update) over its average launch duration, timed with HIP
                 events on the plan's stream, against the 8 TB/s HBM peak;
   cpu_baseline  the single-threaded FP32 CPU oracle on a bounded sample of the
-                same grid (a few steps), timed on this host.
+                same grid (a few steps), timed on this host; "all_cores"
+                gives the same sample on up to 16 threads.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--tblock T]
                   [--cpu-steps S] [--no-cpu] [--traffic-file F]
@@ -379,7 +380,16 @@ def main() -> int:
             "sample": f"FP32 CPU oracle, {nx}x{ny}, {args.cpu_steps} steps ({cs:.1f} s, single thread)",
         }
         ref.close()
-        log(f"CPU oracle: {cpu['value']} Gcell/s")
+        # the same sample on the host cores a one-GPU job gets (rows split
+        # over threads, bitwise the single-threaded result)
+        nthr = max(1, min(16, os.cpu_count() or 1))
+        ref = Oracle(setup, precision=32, threads=nthr)
+        c0 = time.perf_counter()
+        ref.run(args.cpu_steps)
+        cs_all = time.perf_counter() - c0
+        ref.close()
+        cpu["all_cores"] = {"value": round(cells * args.cpu_steps / cs_all / 1e9, 4), "cores": nthr}
+        log(f"CPU oracle: {cpu['value']} Gcell/s single-threaded, {cpu['all_cores']['value']} on {nthr} threads")
 
     line = {
         "metric": "Gcell-updates/s",
