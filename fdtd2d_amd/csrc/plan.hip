@@ -209,6 +209,20 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
     }
     return 0;
   };
+  // every cell of the window has Ne > 0: each of its rows has a plasma run
+  // covering the window's columns
+  auto plasma_full = [&](const int4 &w) {
+    if (!m || !m->ne_run_off) return false;
+    int jl0, jl1, cs, ce;
+    window(w.x, w.y, w.z, jl0, jl1, cs, ce);
+    for (int r = jl0; r < jl1; ++r) {
+      bool in = false;
+      for (int q = m->ne_run_off[r]; q < m->ne_run_off[r + 1] && !in; ++q)
+        in = m->ne_runs[q].x <= cs && m->ne_runs[q].y >= ce;
+      if (!in) return false;
+    }
+    return true;
+  };
   std::vector<int4> *lists = t.lists;
   for (int tx = 0; tx < t.ntx; ++tx)
     for (int ty = 0; ty < t.nty; ++ty) {
@@ -258,13 +272,17 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
     });
     for (int4 &w : lists[c]) {
       w.w = c == 0 ? needs_jz(w) : 1;
+      if (c == 0 && w.w && plasma_full(w)) w.w |= 2;
       const long long c0 = c == 0 ? t.x0 + static_cast<long long>(w.x) * t.vw : edge_entry_c0(w.x);
       const long long c1 = c == 0 ? std::min<long long>(nx, c0 + t.vw) : c0 + edge_entry_w(w.x);
       t.cells[c] += static_cast<double>(std::max<long long>(0, c1 - c0)) * (w.z - w.y);
     }
     t.n_tiles[c] = static_cast<int>(lists[c].size());
     if (c == 0)
-      for (const int4 &w : lists[c]) t.n_nojz += w.w == 0;
+      for (const int4 &w : lists[c]) {
+        t.n_nojz += w.w == 0;
+        t.n_full += (w.w & 2) != 0;
+      }
     while (lists[c].size() % fdtd2d::kWavesPerWG) lists[c].push_back(make_int4(0, 0, 0, 0));
   }
   return t;
@@ -306,6 +324,10 @@ int set_tiling(fdtd2d_plan *pl) {
   m.ne_lo = pl->ne_lo.data();
   m.ne_hi = pl->ne_hi.data();
   m.jz_lo = pl->jz_lo.data();
+  if (!pl->ne_run_off.empty()) {
+    m.ne_run_off = pl->ne_run_off.data();
+    m.ne_runs = pl->ne_runs.data();
+  }
   m.jz_hi = pl->jz_hi.data();
   m.ev = pl->ev_cells.data();
   m.nev = pl->ev_cells.size();
@@ -326,6 +348,7 @@ int set_tiling(fdtd2d_plan *pl) {
     a.n_tiles[c] = t.n_tiles[c];
   }
   pl->n_nojz = t.n_nojz;
+  pl->n_full = t.n_full;
   if (pl->pooled) {  // the pool concatenates the host lists of its shots
     a.n_wg[0] = static_cast<int>(t.lists[0].size()) / fdtd2d::kWavesPerWG;
     a.n_wg[1] = static_cast<int>(t.lists[1].size()) / fdtd2d::kWavesPerWG;
@@ -691,15 +714,25 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   pl->ne_hi.assign(ny, -1);
   pl->jz_lo.assign(ny, INT32_MAX);
   pl->jz_hi.assign(ny, -1);
-  for (int j = 0; j < ny; ++j)
+  pl->ne_run_off.assign(1, 0);
+  pl->ne_runs.clear();
+  for (int j = 0; j < ny; ++j) {
+    int run = -1;  // first column of the current run of Ne > 0
     for (int i = 0; i < nx; ++i) {
       const size_t c = static_cast<size_t>(j) * nx + i;
       const float ne = p->medium_kind == FDTD2D_MEDIUM_TABLE ? p->ne[p->material[c]] : p->density[c];
       if (ne > 0.f) {
         pl->ne_lo[j] = std::min(pl->ne_lo[j], i);
         pl->ne_hi[j] = std::max(pl->ne_hi[j], i);
+        if (run < 0) run = i;
+      } else if (run >= 0) {
+        pl->ne_runs.push_back(make_int2(run, i));
+        run = -1;
       }
     }
+    if (run >= 0) pl->ne_runs.push_back(make_int2(run, nx));
+    pl->ne_run_off.push_back(static_cast<int>(pl->ne_runs.size()));
+  }
   if ((rc = upload(pl, L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
   if ((rc = upload(pl, L.off_events, events.data(), events.size() * sizeof(fdtd2d::Event)))) return fail(rc);
   // RC tracker: probe index per column, per-line probe pair, edge levels
@@ -877,6 +910,7 @@ int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value)
   else if (n == "tiles_interior") *value = pl->a.n_tiles[0];
   else if (n == "tiles_cpml") *value = pl->a.n_tiles[1];
   else if (n == "tiles_nojz") *value = pl->n_nojz;
+  else if (n == "tiles_plasma_full") *value = pl->n_full;
   else if (n == "cells_interior") *value = pl->cells_class[0];
   else if (n == "cells_cpml") *value = pl->cells_class[1];
   else if (n == "tile_cols") *value = pl->a.vw;

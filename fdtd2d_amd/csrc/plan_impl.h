@@ -68,12 +68,17 @@ struct Tiling {
   std::vector<int4> lists[2];  // padded to whole work-groups
   int n_tiles[2] = {0, 0};
   int n_nojz = 0;  // interior tiles that skip the Jz update
+  int n_full = 0;  // interior tiles whose window is full of plasma
   double cells[2] = {0, 0};
 };
 
 struct TileMasks {
   const int *srow_pref = nullptr, *scol_pref = nullptr;  // [ny+1], [nx+1]
   const int *ne_lo = nullptr, *ne_hi = nullptr, *jz_lo = nullptr, *jz_hi = nullptr;  // [ny]
+  // per row, the runs [x, y) of consecutive columns with Ne > 0: row r's
+  // runs are ne_runs[ne_run_off[r] .. ne_run_off[r+1])
+  const int *ne_run_off = nullptr;  // [ny+1]
+  const int2 *ne_runs = nullptr;
   const int2 *ev = nullptr;  // event cells {column, row}, sorted by (row, column)
   size_t nev = 0;
 };
@@ -107,6 +112,7 @@ struct fdtd2d_plan {
   double cells_class[2] = {0, 0};  // grid cells owned by interior / CPML-class tiles
   int hc = 0;                      // rows per CPML-class sub-band in use
   int n_nojz = 0;                  // interior tiles skipping the Jz update
+  int n_full = 0;                  // interior tiles whose window is full of plasma
   int4 *wg_buf = nullptr;          // device copy of the two work-group lists
   hipStream_t aux = nullptr;       // CPML-class launches run here, beside the interior ones
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -124,6 +130,10 @@ struct fdtd2d_plan {
   // update (there Jz stays +0 and Ez - Jz = Ez).
   std::vector<int> srow_pref, scol_pref;
   std::vector<int> ne_lo, ne_hi, jz_lo, jz_hi;
+  // runs of plasma columns per row (see TileMasks): an interior tile whose
+  // whole window lies inside them skips the Ne > 0 test (flag bit 1)
+  std::vector<int> ne_run_off;
+  std::vector<int2> ne_runs;
   // cells holding a source, detector or probe, sorted by (row, column): the
   // tiles whose window holds one take the CPML-class kernel, so the interior
   // kernel carries no event code

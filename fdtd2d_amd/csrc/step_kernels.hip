@@ -166,7 +166,7 @@ __device__ __forceinline__ RowS<EDGE> row_scalars(const StepArgs &a, int r) {
 
 // One level: level t-1's row r (p), Ez^{t-1}(r+1) (ez_s) and Hx^{t-1/2}(r-1)
 // (hx_n) -> level t's row r (o).
-template <int C, bool EDGE, int MED, bool JZ, bool EV, bool XT>
+template <int C, bool EDGE, int MED, int JZ, bool EV, bool XT>
 __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl,
                                         const RowS<EDGE> &rs, const Level<C, EDGE> &p, const float (&ez_s)[C],
                                         const float (&hx_n)[C], int r, int step, int own0, int own1,
@@ -246,10 +246,12 @@ __device__ __forceinline__ void advance(const StepArgs &a, const float4 *lut_s, 
     else den_ex = a.uden_ex;
     const float curl = dHy * den_ex + dHx * rs.den_ey;
     float jz = 0.f, ez;
-    if constexpr (JZ) {
+    if constexpr (JZ != 0) {
       // Jz^{n+1/2} from Ez^n where Ne > 0; (...) - Jz as the reference orders it
       const float jz_new = jz_update(p.jz[c], a.jz_decay, a.cj, p.ez[c], ne, a.jz_dt);
-      jz = (updE && ne > 0.f) ? jz_new : p.jz[c];
+      // JZ == 2: every cell of the tile's window has Ne > 0 (checked on the
+      // host), so the test always picks the update
+      jz = (JZ == 2 || (updE && ne > 0.f)) ? jz_new : p.jz[c];
       ez = ca * p.ez[c] + cb * (curl - jz);
     } else {
       // the window holds no plasma and Jz = +0, so curl - Jz = curl exactly
@@ -413,7 +415,7 @@ __device__ __forceinline__ void store_row(float *base_, long long rowp, int c0, 
 
 // Level 0 of row k: the step-n0 state (zeros for rows outside [0, ny) or
 // past the band's last loaded row).
-template <int C, bool EDGE, int MED, bool JZ>
+template <int C, bool EDGE, int MED, int JZ>
 __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm, const Cols<C, EDGE> &cl,
                                             int src, int k, bool rv, bool lane_in, Level<C, EDGE> &in) {
   const int pitch = a.pitch;
@@ -429,13 +431,13 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
     oload<C>(sm.ez, off, in.ez);
     oload<C>(sm.hx, off, in.hx);
     oload<C>(sm.hy, off, in.hy);
-    if constexpr (JZ) {
+    if constexpr (JZ != 0) {
       oload<C>(sm.jz, off, in.jz);
     } else {
 #pragma unroll
       for (int c = 0; c < C; ++c) in.jz[c] = 0.f;
     }
-    if constexpr (JZ || MED == kTable) {
+    if constexpr (JZ != 0 || MED == kTable) {
       oload<C>(sm.med, off, mv);
 #pragma unroll
       for (int c = 0; c < C; ++c) in.med[c] = __float_as_uint(mv[c]);
@@ -485,7 +487,7 @@ __device__ __forceinline__ void load_level0(const StepArgs &a, const Streams &sm
 // level and all later ones are skipped. A skipped level leaves its pipeline
 // register as it was; the first needed row of a level reads only its Hx
 // from that register's rows, and only for an Ez nothing needs.
-template <int T, int C, bool EDGE, int MED, bool JZ, bool EV, bool XT, bool WARM>
+template <int T, int C, bool EDGE, int MED, int JZ, bool EV, bool XT, bool WARM>
 __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, const Cols<C, EDGE> &cl, int n0,
                                        int k, int jb0, int jb1, int own0, int own1, Level<C, EDGE> prev,
                                        Level<C, EDGE> (&st)[T + 1]) {
@@ -511,7 +513,7 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
 
 // One row iteration: level 0 takes row k, level t computes row k-t, and
 // row k-T is stored.
-template <int T, int C, bool EDGE, int MED, bool JZ, bool XT, bool WARM>
+template <int T, int C, bool EDGE, int MED, int JZ, bool XT, bool WARM>
 __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, const float4 *lut_s,
                                          const Cols<C, EDGE> &cl, int src, int n0, int k, int jb0, int jb1,
                                          int own0, int own1, bool lane_owns, unsigned trash_off,
@@ -533,7 +535,7 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
     ostore<C>(dst.ez, off, o.ez);
     ostore<C>(dst.hx, off, o.hx);
     ostore<C>(dst.hy, off, o.hy);
-    if constexpr (JZ) ostore<C>(dst.jz, off, o.jz);
+    if constexpr (JZ != 0) ostore<C>(dst.jz, off, o.jz);
   } else if (r >= jb0 && r < jb1) {
     const int pitch = a.pitch;
     const Level<C, EDGE> &o = st[T];
@@ -571,7 +573,7 @@ __device__ __forceinline__ bool extras_on(const StepArgs &a) {
 // is indexed statically.
 // x: the interior entry's tile index (columns [x*vw, x*vw+vw)), or the
 // CPML-class entry's packed first column and width (one column per lane).
-template <int T, int C, int D, bool EDGE, int MED, bool JZ, bool XT = false>
+template <int T, int C, int D, bool EDGE, int MED, int JZ, bool XT = false>
 __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s, int src, int n0,
                                          int x, int jb0, int jb1) {
   static_assert(!EDGE || C == 1, "CPML-class tiles run one column per lane");
@@ -707,7 +709,16 @@ __global__ void __launch_bounds__(64 * kWavesPerWG) step_kernel(const StepArgs a
     else
       run_tile<T, C, D, true, MED, true, false>(a, lut_s, src, n0, tx, r0, r1);
   } else if (flags & 1) {
-    run_tile<T, C, D, false, MED, true>(a, lut_s, src, n0, tx, r0, r1);
+    // tiles full of plasma skip the Ne > 0 test (two columns per lane and a
+    // density medium: the large grids)
+    if constexpr (C >= 2 && MED == kDensity) {
+      if (flags & 2)
+        run_tile<T, C, D, false, MED, 2>(a, lut_s, src, n0, tx, r0, r1);
+      else
+        run_tile<T, C, D, false, MED, 1>(a, lut_s, src, n0, tx, r0, r1);
+    } else {
+      run_tile<T, C, D, false, MED, 1>(a, lut_s, src, n0, tx, r0, r1);
+    }
   } else {
     run_tile<T, C, D, false, MED, false>(a, lut_s, src, n0, tx, r0, r1);
   }

@@ -294,7 +294,10 @@ int fdtd2d_plan_set_tblock(fdtd2d_plan *plan, int steps_per_launch);
 int fdtd2d_plan_set_option(fdtd2d_plan *plan, const char *name, double value);
 /* Plan facts by name: "pitch", "tblock", "band", "band_cpml", "ntx", "nty",
    "workgroups", "workgroups_cpml", "tile_cols", "bytes", "steps",
-   "cols_per_lane", "prefetch", "jz_first", "cells_interior", "cells_cpml". */
+   "cols_per_lane", "prefetch", "jz_first", "cells_interior", "cells_cpml",
+   "tiles_interior", "tiles_cpml", "tiles_nojz" (interior tiles with no Jz
+   update), "tiles_plasma_full" (interior tiles whose window has Ne > 0 in
+   every cell). */
 int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
 /* Advance nsteps steps (asynchronous on the plan's stream). */
 int fdtd2d_run(fdtd2d_plan *plan, int nsteps);

@@ -56,7 +56,8 @@ def test_plain_400_full_length(gpu_device, tblock, cols, depth):
         plan.set_option("tblock", tblock)
         plan.set_option("prefetch", depth)
         print(f"T={tblock} C={cols} D={depth} band={plan.info('band')} tiles={plan.info('ntx')}x{plan.info('nty')} "
-              f"cpml wgs={plan.info('workgroups_cpml')}/{plan.info('workgroups')}", flush=True)
+              f"cpml wgs={plan.info('workgroups_cpml')}/{plan.info('workgroups')} "
+              f"plasma-full tiles={plan.info('tiles_plasma_full')}", flush=True)
         plan.run(2000)
         plan.sync()
         gstate = all_state(plan, 12)
@@ -130,7 +131,7 @@ def _run_pair(p, gpu_device, opts, label, prepare=None):
             plan.set_option(k, v)
         if prepare:
             prepare(ref, plan)
-        info = {k: plan.info(k) for k in ("tiles_interior", "tiles_cpml", "tiles_nojz")}
+        info = {k: plan.info(k) for k in ("tiles_interior", "tiles_cpml", "tiles_nojz", "tiles_plasma_full")}
         print(f"{label}: {p.nx}x{p.ny} P={p.npml} {opts} {info}", flush=True)
         ref.run(p.nsteps)
         plan.run(p.nsteps)
@@ -156,6 +157,19 @@ def test_interior_paths(gpu_device, seed):
             "band_cpml": int(rng.choice([0, 8, 32]))}
     info = _run_pair(p, gpu_device, opts, f"interior seed {seed}")
     assert info["tiles_interior"] > 0
+
+
+@pytest.mark.parametrize("tblock,depth", [(8, 2), (3, 1)])
+def test_plasma_full_tiles(gpu_device, tblock, depth):
+    """Interior tiles whose whole window holds plasma (two columns per lane,
+    density medium) run without the Ne > 0 test; the tiles across the ramp's
+    foot and the plasma-free ones keep their variants."""
+    rng = np.random.default_rng(91 + tblock)
+    p = random_problem(rng, nx=900, ny=260, npml=8, nsteps=26, table=False)
+    info = _run_pair(p, gpu_device, {"cols_per_lane": 2, "tblock": tblock, "prefetch": depth}, "plasma full")
+    assert info["tiles_plasma_full"] > 0
+    assert info["tiles_nojz"] > 0
+    assert info["tiles_interior"] > info["tiles_plasma_full"] + info["tiles_nojz"]
 
 
 def test_jz_written_where_no_plasma(gpu_device):
