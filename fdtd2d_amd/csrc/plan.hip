@@ -127,14 +127,15 @@ int hip_fail(hipError_t e, const char *what) {
 
 // Band height: enough bands for >= ~8 waves per SIMD on 256 CUs, at least 32
 // rows, at most 256. With two columns per lane (a wider tile, about twice
-// the state per wave) about 6 waves per SIMD, from a finer set of heights:
-// 8192^2 then takes 96-row bands (measured 752 Gcell/s against 703 at 128
-// and 735 at 64).
+// the state per wave) about 6 waves per SIMD, from a finer set of heights
+// up to 192: 8192^2 then takes 96-row bands (measured 752 Gcell/s against
+// 703 at 128 and 735 at 64), 16384^2 192-row bands (800 against 744 at 256,
+// 757 at 128 and 781 at 96).
 int pick_band(int nx, int ny, int VW, int T, int C) {
   const int ntx = (nx + VW - 1) / VW;
   auto waves = [&](int H) { return static_cast<long long>(ntx) * ((ny + H - 1) / H); };
   if (C >= 2) {
-    static const int heights[] = {256, 192, 160, 128, 112, 96, 80, 64, 48, 32};
+    static const int heights[] = {192, 160, 128, 112, 96, 80, 64, 48, 32};
     for (int H : heights)
       if (waves(H) >= 6LL * 1024) return std::max(H, T);
     return std::max(32, T);

@@ -81,6 +81,9 @@ def test_partition_8192_two_columns():
     assert cells[1] / (8192 * 8192) < 0.03
     mid = sorted({(c0, c1) for c0, c1, r0, _, c in ent if c == 1 and 200 < r0 < 7900})
     assert all(c1 <= 96 or c0 >= 8192 - 200 for c0, c1 in mid)
+    # the default bands at two columns per lane
+    assert _lib.tile_lists(8192, 8192, 16, 0, 2, 0, 0)[1]["band"] == 96
+    assert _lib.tile_lists(16384, 16384, 16, 0, 2, 0, 0)[1]["band"] == 192
 
 
 def test_event_cells_take_cpml_class():
