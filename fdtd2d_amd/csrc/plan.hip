@@ -225,6 +225,10 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
     return true;
   };
   std::vector<int4> *lists = t.lists;
+  // rows [ytop, ybot) are where a band's window (T rows of cone above and
+  // below) can stay clear of the y strips and the ring
+  const int Pe = std::max(P, 1);
+  const int ytop = Pe + T + 1, ybot = ny - Pe - T - 2;
   for (int tx = 0; tx < t.ntx; ++tx)
     for (int ty = 0; ty < t.nty; ++ty) {
       const int jb0 = ty * t.H, jb1 = std::min(ny, jb0 + t.H);
@@ -233,8 +237,15 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
         continue;
       }
       int run0 = -1;  // start of the current run of interior sub-bands
-      for (int r0 = jb0; r0 < jb1; r0 += t.hc) {
-        const int r1 = std::min(jb1, r0 + t.hc);
+      for (int r0 = jb0, r1; r0 < jb1; r0 = r1) {
+        r1 = std::min(jb1, r0 + t.hc);
+        // sub-bands end at the first row whose window clears the top strip
+        // and at the last one clear of the bottom strip, so the rows between
+        // them and the strips can run as interior tiles (when the band keeps
+        // at least a sub-band's height of them: shorter interior tiles cost
+        // more in cone rows than they save)
+        if (r0 < ytop && r1 > ytop && jb1 - ytop >= t.hc) r1 = ytop;
+        if (r0 < ybot && r1 > ybot && ybot - jb0 >= t.hc) r1 = ybot;
         if (touches(tx, r0, r1)) {
           if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, r0, 0));
           run0 = -1;
