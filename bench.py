@@ -8,9 +8,16 @@ region, which brackets exactly --steps steps with device synchronisation on
 both sides.
 
 Prints ONE JSON line (rank 0). Besides the metric it carries
-  roofline      the step kernel's algorithmic bytes per launch (36 B per cell
-                update) over its average launch duration, timed with HIP
-                events on the plan's stream, against the 8 TB/s HBM peak;
+  roofline      the interior step kernel against the 8 TB/s HBM peak. A
+                launch advances its cells T steps in registers but must move
+                each cell's state through HBM once: 36 B per cell (read and
+                write Ez, Hx, Hy, Jz; read the 4-byte medium word).
+                `achieved` = 36 B x interior cells x launches / summed kernel
+                time (HIP events on the plan's stream), `frac` = achieved /
+                peak, `traffic` = counter-measured HBM bytes per launch.
+                `step_equiv_gbs` = 36 B x interior cells x steps / the same
+                time: the rate a kernel moving every step's state through
+                HBM would need (above the peak: temporal blocking);
   cpu_baseline  the single-threaded FP32 CPU oracle on a bounded sample of the
                 same grid (a few steps), timed on this host; "all_cores"
                 gives the same sample on up to 16 threads.
@@ -47,6 +54,32 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 36  # r+w of Ez, Hx, Hy, Jz + one 4-byte medium word
+
+
+def host_cpu() -> dict:
+    """CPU model and logical CPU count of this host (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": affinity, "cpu_count": os.cpu_count()}
+
+
+def trace_check(det) -> tuple:
+    """(ok, message): every detector trace finite and not all zero."""
+    import numpy as np
+
+    finite = bool(np.isfinite(det).all())
+    zero = int((np.abs(det[1:]).max(axis=0) == 0).sum()) if det.size else 0
+    return finite and zero == 0, f"finite: {finite}, all-zero detector traces: {zero} of {det.shape[1]}"
 
 
 def log(msg: str) -> None:
@@ -182,6 +215,9 @@ def bench_batch(args) -> int:
             nonfinite += int((~np.isfinite(d)).sum())
         log(f"plain loop: {tp:.3f}s, {cells / tp / 1e9:.1f} Gcell/s; batched traces bitwise equal: {same}; "
             f"non-finite trace values: {nonfinite}")
+    bad = [k for k, d in enumerate(det_b) if not trace_check(d)[0]]
+    if bad:
+        log(f"batch: shots {bad[:10]} have non-finite or all-zero detector traces")
 
     cpu = None
     if not args.no_cpu:
@@ -222,10 +258,11 @@ def bench_batch(args) -> int:
                                           "gcell_per_s": round(cells / tp / 1e9, 2)},
         "speedup_vs_plain": None if tp is None else round(tp / tb, 3),
         "traces_bitwise_equal_to_plain": same,
+        "traces_checked": {"shots": len(det_b), "failed": len(bad)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    return 0 if same is not False else 1
+    return 0 if same is not False and not bad else 1
 
 
 def bench_serve(args) -> int:
@@ -342,18 +379,25 @@ def main() -> int:
     plan.run(args.steps)
     plan.sync()
     elapsed = time.perf_counter() - t_start
-    avg_ms, nlaunch = plan.kernel_ms()
+    avg_ms, nlaunch = plan.kernel_ms(0)
+    cpml_ms, _ = plan.kernel_ms(1)
     cells_int = plan.info("cells_interior")
+    cells_cpml = plan.info("cells_cpml")
     plan_c = plan.info("cols_per_lane")
     det, _ = plan.read_detectors()
     plan.close()
-    ok = bool((det == det).all())  # no NaN in the traces
+    # only the rows of the steps run so far can hold samples
+    ok, check_msg = trace_check(det[: args.warmup + args.steps + 1])
     cells = nx * ny
     value = cells * args.steps / elapsed / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
-    # the timed kernel is the interior one: T steps over the interior cells
-    full_launch_updates = cells_int * T
-    achieved = BYTES_PER_CELL_UPDATE * full_launch_updates / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # The timed kernel is the interior one. The run splits --steps into
+    # launches of min(T, steps left); each launch moves the state of its
+    # cells through HBM once, whatever its step count.
+    kernel_s = avg_ms * nlaunch * 1e-3
+    launch_bytes = BYTES_PER_CELL_UPDATE * cells_int
+    achieved = launch_bytes * nlaunch / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    step_equiv = BYTES_PER_CELL_UPDATE * cells_int * args.steps / kernel_s / 1e9 if kernel_s > 0 else 0.0
     traffic = None
     tf = Path(args.traffic_file)
     if tf.exists():
@@ -364,8 +408,9 @@ def main() -> int:
                 traffic = rec.get("bytes_per_launch")
         except Exception:
             traffic = None
-    log(f"GPU: {value:.1f} Gcell/s, {ms_per_step:.3f} ms/step, kernel avg {avg_ms:.3f} ms over {nlaunch} "
-        f"launches of T={T} ({cells_int / cells:.3f} of the grid), finite traces: {ok}")
+    log(f"GPU: {value:.1f} Gcell/s, {ms_per_step:.3f} ms/step, interior kernel avg {avg_ms:.4f} ms over {nlaunch} "
+        f"launches (T={T}, {cells_int / cells:.3f} of the grid): {achieved:.0f} GB/s of one pass per launch "
+        f"(frac {achieved / HBM_PEAK_GBS:.3f}), single-step equivalent {step_equiv:.0f} GB/s; traces {check_msg}")
 
     cpu = None
     if not args.no_cpu:
@@ -375,17 +420,21 @@ def main() -> int:
         c0 = time.perf_counter()
         ref.run(args.cpu_steps)
         cs = time.perf_counter() - c0
+        hc = host_cpu()
         cpu = {
             "value": round(cells * args.cpu_steps / cs / 1e9, 4),
             "unit": "Gcell-updates/s",
             "cores": 1,
             "kind": "port",
             "sample": f"FP32 CPU oracle, {nx}x{ny}, {args.cpu_steps} steps ({cs:.1f} s, single thread)",
+            "cpu_model": hc["model"],
+            "nproc": hc["nproc"],
         }
         ref.close()
         # the same sample on the host cores a one-GPU job gets (rows split
         # over threads, bitwise the single-threaded result)
-        nthr = max(1, min(16, os.cpu_count() or 1))
+        # every CPU this process may run on (the box gives a one-GPU job 16)
+        nthr = max(1, hc["nproc"])
         ref = Oracle(setup, precision=32, threads=nthr)
         c0 = time.perf_counter()
         ref.run(args.cpu_steps)
@@ -422,14 +471,22 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_frac": None if traffic is None else round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel": f"interior step kernel (T={T}, {int(plan_c)} columns per lane)",
             "kernel_avg_ms": round(avg_ms, 4),
-            "kernel": "interior step kernel",
+            "launches": nlaunch,
             "cells_per_launch": int(cells_int),
-            "bytes_per_launch": int(BYTES_PER_CELL_UPDATE * full_launch_updates),
+            "bytes_per_launch": int(launch_bytes),
+            "step_equiv_gbs": round(step_equiv, 1),
+            "cpml_kernel": {"avg_ms": round(cpml_ms, 4), "cells_per_launch": int(cells_cpml),
+                            "gbs_one_pass": round(BYTES_PER_CELL_UPDATE * cells_cpml / (cpml_ms * 1e-3) / 1e9, 1)
+                            if cpml_ms > 0 else None},
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if not ok:
+        log(f"bench: detector traces failed the check ({check_msg})")
     return 0 if ok else 1
 
 

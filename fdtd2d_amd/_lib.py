@@ -181,6 +181,7 @@ SIGNATURES = [
     ("fdtd2d_plan_destroy", C.c_int, [C.c_void_p]),
     ("fdtd2d_plan_time_kernels", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_plan_kernel_ms", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    ("fdtd2d_plan_kernel_time", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     ("fdtd2d_pool_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     ("fdtd2d_pool_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     ("fdtd2d_pool_get_info", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),

@@ -398,16 +398,18 @@ int upload(fdtd2d_plan *pl, size_t off, const void *src, size_t bytes) {
 }
 
 int drain_timing(fdtd2d_plan *pl) {
-  for (auto &pr : pl->pending) {
-    float ms = 0;
-    HIP_TRY(hipEventSynchronize(pr.second), "hipEventSynchronize");
-    HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second), "hipEventElapsedTime");
-    pl->sum_ms += ms;
-    pl->count += 1;
-    pl->spare.push_back(pr.first);
-    pl->spare.push_back(pr.second);
+  for (int c = 0; c < 2; ++c) {
+    for (auto &pr : pl->pending[c]) {
+      float ms = 0;
+      HIP_TRY(hipEventSynchronize(pr.second), "hipEventSynchronize");
+      HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second), "hipEventElapsedTime");
+      pl->sum_ms[c] += ms;
+      pl->count[c] += 1;
+      pl->spare.push_back(pr.first);
+      pl->spare.push_back(pr.second);
+    }
+    pl->pending[c].clear();
   }
-  pl->pending.clear();
   return FDTD2D_OK;
 }
 
@@ -944,28 +946,38 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
   const int Tmax = current_T(pl);
   while (nsteps > 0) {
     const int t = std::min(Tmax, nsteps);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, c0 = nullptr, c1 = nullptr;
     if (pl->timing) {
       e0 = get_event(pl);
       e1 = get_event(pl);
-      if (!e0 || !e1) return set_error(FDTD2D_EDEVICE, "hipEventCreate");
+      c0 = get_event(pl);
+      c1 = get_event(pl);
+      if (!e0 || !e1 || !c0 || !c1) return set_error(FDTD2D_EDEVICE, "hipEventCreate");
     }
     // CPML work-groups on the aux stream, interior ones on the main stream;
     // both read buffer set src and write the other, so they may overlap.
     if (pl->a.n_wg[1] > 0) {
       HIP_TRY(hipEventRecord(pl->ev_fork, pl->stream), "hipEventRecord fork");
       HIP_TRY(hipStreamWaitEvent(pl->aux, pl->ev_fork, 0), "hipStreamWaitEvent fork");
+      if (pl->timing) HIP_TRY(hipEventRecord(c0, pl->aux), "hipEventRecord");
       hipError_t e = fdtd2d::launch_step(pl->a, 1, t, pl->C, 1, pl->src, pl->n, pl->aux);
       if (e != hipSuccess) return hip_fail(e, "CPML step kernel launch");
+      if (pl->timing) {
+        HIP_TRY(hipEventRecord(c1, pl->aux), "hipEventRecord");
+        pl->pending[1].push_back({c0, c1});
+      }
       HIP_TRY(hipEventRecord(pl->ev_join, pl->aux), "hipEventRecord join");
+    } else if (pl->timing) {
+      pl->spare.push_back(c0);
+      pl->spare.push_back(c1);
     }
-    // timing events bracket the interior kernel alone on the main stream
+    // timing events bracket each kernel alone on its own stream
     if (pl->timing) HIP_TRY(hipEventRecord(e0, pl->stream), "hipEventRecord");
     hipError_t e = fdtd2d::launch_step(pl->a, 0, t, pl->C, pl->D, pl->src, pl->n, pl->stream);
     if (e != hipSuccess) return hip_fail(e, "step kernel launch");
     if (pl->timing) {
       HIP_TRY(hipEventRecord(e1, pl->stream), "hipEventRecord");
-      pl->pending.push_back({e0, e1});
+      pl->pending[0].push_back({e0, e1});
     }
     if (pl->a.n_wg[1] > 0) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
     if (pl->nrc > 0) {
@@ -1148,28 +1160,36 @@ int fdtd2d_plan_time_kernels(fdtd2d_plan *pl, int enable) {
   int rc = fdtd2d_sync(pl);
   if (rc) return rc;
   pl->timing = enable != 0;
-  pl->sum_ms = 0;
-  pl->count = 0;
+  for (int c = 0; c < 2; ++c) {
+    pl->sum_ms[c] = 0;
+    pl->count[c] = 0;
+  }
+  return FDTD2D_OK;
+}
+
+int fdtd2d_plan_kernel_time(fdtd2d_plan *pl, int cls, double *avg_ms, int *count) {
+  if (!pl || !avg_ms || !count) return set_error(FDTD2D_EINVAL, "null argument");
+  if (cls != 0 && cls != 1) return set_error(FDTD2D_EINVAL, "kernel class must be 0 or 1");
+  int rc = fdtd2d_sync(pl);
+  if (rc) return rc;
+  *count = pl->count[cls];
+  *avg_ms = pl->count[cls] ? pl->sum_ms[cls] / pl->count[cls] : 0.0;
   return FDTD2D_OK;
 }
 
 int fdtd2d_plan_kernel_ms(fdtd2d_plan *pl, double *avg_ms, int *count) {
-  if (!pl || !avg_ms || !count) return set_error(FDTD2D_EINVAL, "null argument");
-  int rc = fdtd2d_sync(pl);
-  if (rc) return rc;
-  *count = pl->count;
-  *avg_ms = pl->count ? pl->sum_ms / pl->count : 0.0;
-  return FDTD2D_OK;
+  return fdtd2d_plan_kernel_time(pl, 0, avg_ms, count);
 }
 
 int fdtd2d_plan_destroy(fdtd2d_plan *pl) {
   if (!pl) return FDTD2D_OK;
   hipSetDevice(pl->device);
   if (pl->stream) hipStreamSynchronize(pl->stream);
-  for (auto &pr : pl->pending) {
-    hipEventDestroy(pr.first);
-    hipEventDestroy(pr.second);
-  }
+  for (int c = 0; c < 2; ++c)
+    for (auto &pr : pl->pending[c]) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
   for (auto e : pl->spare) hipEventDestroy(e);
   if (pl->aux) hipStreamSynchronize(pl->aux);
   if (pl->ev_fork) hipEventDestroy(pl->ev_fork);

@@ -118,10 +118,12 @@ struct fdtd2d_plan {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int ndet = 0, ncomp = 0, nsrc = 0;
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  // timed launches per kernel class (0 interior on the plan's stream, 1
+  // CPML class on the aux stream): event pairs not yet read, and the totals
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
   std::vector<hipEvent_t> spare;
-  double sum_ms = 0;
-  int count = 0;
+  double sum_ms[2] = {0, 0};
+  int count[2] = {0, 0};
   int launches_T[fdtd2d::kMaxT + 1] = {0};
   // Host facts the tiling needs. Prefix counts of the rows / columns whose
   // den values differ from the uniform interior ones (the interior kernel

@@ -427,9 +427,11 @@ class Plan:
     def time_kernels(self, enable: bool = True) -> None:
         L.check(L.lib().fdtd2d_plan_time_kernels(self._h, 1 if enable else 0), "fdtd2d_plan_time_kernels")
 
-    def kernel_ms(self):
+    def kernel_ms(self, cls: int = 0):
+        """(average ms, launches timed) of the interior (cls 0) or CPML-class
+        (cls 1) step kernel since time_kernels(True)."""
         ms, n = C.c_double(), C.c_int()
-        L.check(L.lib().fdtd2d_plan_kernel_ms(self._h, C.byref(ms), C.byref(n)), "fdtd2d_plan_kernel_ms")
+        L.check(L.lib().fdtd2d_plan_kernel_time(self._h, cls, C.byref(ms), C.byref(n)), "fdtd2d_plan_kernel_time")
         return ms.value, n.value
 
     def close(self) -> None:

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FDTD2D_ABI_VERSION 3
+#define FDTD2D_ABI_VERSION 4
 
 /* status codes */
 enum {
@@ -346,6 +346,12 @@ int fdtd2d_plan_destroy(fdtd2d_plan *plan);
  * 0 disables timing. */
 int fdtd2d_plan_time_kernels(fdtd2d_plan *plan, int enable);
 int fdtd2d_plan_kernel_ms(fdtd2d_plan *plan, double *avg_ms, int *count);
+/* The same for either kernel class: cls 0 the interior kernel (as
+ * fdtd2d_plan_kernel_ms), 1 the CPML-class kernel (the work-groups whose
+ * windows touch a CPML strip, the outer ring or an event cell, `cells_cpml`
+ * of fdtd2d_plan_get_info; timed on the plan's second stream, where it runs
+ * beside the interior kernel). count = launches timed. */
+int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *count);
 
 /* ------------------------------------------------------------------ pools
  * Many independent shots (any grid size, medium, sources, detectors and step

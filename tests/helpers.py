@@ -92,3 +92,49 @@ def all_state(sim, npml: int):
     out["det"] = np.asarray(det, np.float64)
     out["comp"] = np.asarray(comp, np.float64)
     return out
+
+
+def fp64_check(problem, det32, label: str, threads: int = 4, comp32=None, nsteps=None) -> float:
+    """Runs the FP64 oracle on the same problem (a Problem or a Setup), prints
+    the relative L2 error of the FP32 detector traces (and component traces,
+    if given) against it, and asserts it is within 1e-5 wherever the FP64
+    traces are nonzero. Returns the detector error."""
+    from oracle.oracle import Oracle
+
+    ref = Oracle(problem, 64, threads=threads)
+    ref.run(ref.nsteps if nsteps is None else nsteps)
+    det64, comp64 = ref.read_detectors()
+    ref.close()
+    err = rel_l2(det32, det64)
+    msg = f"{label}: detector rel L2 vs FP64 oracle = {err:.3e} (max |det| {np.abs(det64).max():.3e})"
+    if comp32 is not None and comp64.size:
+        cerr = rel_l2(comp32, comp64)
+        msg += f", components {cerr:.3e}"
+        if np.abs(comp64).max() > 0:
+            assert cerr <= 1e-5, msg
+    print(msg, flush=True)
+    if np.abs(det64).max() > 0:
+        assert err <= 1e-5, msg
+    return err
+
+
+def compare_fields_lean(sim, ref, label: str, npml: int = 0, psi: bool = True) -> None:
+    """Bitwise comparison of every field (and psi strip) of a plan with the
+    FP32 oracle, one array at a time in float32 (for grids whose float64
+    copies would not fit comfortably in host memory)."""
+    names = [("ez", F.EZ), ("hx", F.HX), ("hy", F.HY), ("jz", F.JZ)]
+    for name, w in names:
+        g = np.asarray(sim.read_field(w), np.float32)
+        r = ref.read_field(w).astype(np.float32)
+        if not np.array_equal(g.view(np.uint32), r.view(np.uint32)):
+            bad = np.argwhere(g.view(np.uint32) != r.view(np.uint32))
+            raise AssertionError(f"{label}: {name} differs at {len(bad)} cells, first {bad[:5].tolist()}")
+        del g, r
+    if psi and npml > 0:
+        for name, w in (("psi_hxy", F.PSI_HXY), ("psi_hyx", F.PSI_HYX), ("psi_ezx", F.PSI_EZX),
+                        ("psi_ezy", F.PSI_EZY)):
+            g = np.asarray(sim.read_psi(w), np.float32)
+            r = ref.read_psi(w).astype(np.float32)
+            if not np.array_equal(g.view(np.uint32), r.view(np.uint32)):
+                bad = np.argwhere(g.view(np.uint32) != r.view(np.uint32))
+                raise AssertionError(f"{label}: {name} differs at {len(bad)} entries, first {bad[:5].tolist()}")

@@ -38,7 +38,7 @@ def test_python_binding_covers_the_header():
 
 def test_status_strings_and_version():
     lib = F.lib()
-    assert lib.fdtd2d_abi_version() == 3
+    assert lib.fdtd2d_abi_version() == 4
     assert lib.fdtd2d_strerror(0) == b"ok"
     assert lib.fdtd2d_strerror(L.EINVAL) == b"invalid argument"
     assert lib.fdtd2d_strerror(99) == b"unknown status"

@@ -14,7 +14,7 @@ import pytest
 import fdtd2d_amd as F
 from oracle.oracle import Oracle
 
-from helpers import all_state, random_problem, rel_l2
+from helpers import all_state, fp64_check, random_problem, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,7 @@ def test_random_small_problems(gpu_device, seed):
         plan.sync()
         g = all_state(plan, p.npml)
     _compare_exact(g, all_state(ref, p.npml), f"seed {seed}")
+    fp64_check(p, g["det"], f"seed {seed}", comp32=g["comp"])
 
 
 def test_resume_equals_single_run(gpu_device):
@@ -138,6 +139,8 @@ def _run_pair(p, gpu_device, opts, label, prepare=None):
         plan.sync()
         g = all_state(plan, p.npml)
     _compare_exact(g, all_state(ref, p.npml), label)
+    if prepare is None:  # the FP64 oracle starts from the problem's own (zero) state
+        fp64_check(p, g["det"], label, comp32=g["comp"])
     return info
 
 
@@ -232,6 +235,7 @@ def test_rc_lines_match_oracle(gpu_device, seed):
               flush=True)
         assert np.array_equal(got, want)
         assert np.array_equal(det, ref.read_detectors()[0].astype(np.float32))
+    fp64_check(p, det, f"rc seed {seed}")
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -272,3 +276,4 @@ def test_processing_line_matches_oracle(gpu_device, seed):
     assert np.array_equal(ez, ref.read_field(F.EZ).astype(np.float32))
     if p.line_j == p.src_j[0]:
         assert np.abs(want).max() > 0
+    fp64_check(p, ref.read_detectors()[0].astype(np.float32), f"line seed {seed}")

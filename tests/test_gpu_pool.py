@@ -13,7 +13,7 @@ import pytest
 import fdtd2d_amd as F
 from oracle.oracle import Oracle
 
-from helpers import random_problem
+from helpers import fp64_check, random_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -74,7 +74,9 @@ def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock, fused, depth)
         _drain(pool)
         print({k: pool.info(k) for k in ("launches", "graph_replays", "eager_launches", "tail_launches")})
         for sid, p in zip(ids, probs):
-            _same(_pool_state(pool, sid), _oracle_state(p), f"shot {sid} {p.nx}x{p.ny}")
+            g = _pool_state(pool, sid)
+            _same(g, _oracle_state(p), f"shot {sid} {p.nx}x{p.ny}")
+            fp64_check(p, g["det"], f"pooled shot {sid} {p.nx}x{p.ny}", comp32=g["comp"])
         if chunk:
             assert pool.info("graph_replays") > 0
 
@@ -92,7 +94,9 @@ def test_pool_reshape_midrun_and_late_admission(gpu_device):
         ids.append(pool.add(probs[5]))
         _drain(pool, 3)
         for sid, p in zip(ids, probs):
-            _same(_pool_state(pool, sid), _oracle_state(p), f"shot {sid}")
+            g = _pool_state(pool, sid)
+            _same(g, _oracle_state(p), f"shot {sid}")
+            fp64_check(p, g["det"], f"reshaped pool shot {sid}", comp32=g["comp"])
 
 
 def test_pool_budget_admission_and_slot_reuse(gpu_device):

@@ -1,6 +1,9 @@
-"""The continuous-batching server on the GPU: every shot's result is the same
-alone and in the mix (bit for bit), under memory pressure that forces
-queueing and backfill."""
+"""The continuous-batching server on the GPU: every served shot's result
+equals the FP32 CPU oracle bit for bit (and its detector error against the
+FP64 oracle is within 1e-5), and equals the same shot run alone on a plan,
+under memory pressure that forces queueing and backfill. Shots run 43 steps,
+so every shot ends in a tail launch (fewer than 8 steps left) while other
+shots are being admitted."""
 
 from __future__ import annotations
 
@@ -9,12 +12,16 @@ import pytest
 
 import fdtd2d_amd as F
 from fdtd2d_amd.serve import Server, latency_summary, sweep_requests
+from oracle.oracle import Oracle
+
+from helpers import fp64_check
 
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_equal_alone(gpu_device):
-    reqs = sweep_requests(rate=30.0, duration=0.2, seed=11, steps=40)
+def test_served_results_equal_oracle_and_alone(gpu_device):
+    steps = 43
+    reqs = sweep_requests(rate=30.0, duration=0.2, seed=11, steps=steps)
     assert len(reqs) >= 2
     nshots = sum(len(r.shots) for r in reqs)
     srv = Server(gpu_device, budget_bytes=600_000_000, chunk_launches=2, max_in_flight=4, builders=4,
@@ -22,16 +29,27 @@ def test_served_results_equal_alone(gpu_device):
     srv.keep_results = True
     try:
         done, st = srv.run(reqs, max_seconds=300)
+        tails = srv.pool.info("tail_launches")
     finally:
         srv.close()
-    print(f"{len(reqs)} requests, {nshots} shots, {st}", latency_summary(done), flush=True)
+    print(f"{len(reqs)} requests, {nshots} shots, tail launches {tails:.0f}, {st}", latency_summary(done),
+          flush=True)
     assert len(done) == len(reqs) and st["shots"] == nshots
+    assert tails > 0
+    worst = 0.0
     for rq in reqs:
-        for pos, (w, sd, sh, steps) in enumerate(rq.shots):
-            s = F.Setup.workload(w, seed=sd, shot=sh, steps=steps)
+        for pos, (w, sd, sh, nst) in enumerate(rq.shots):
+            s = F.Setup.workload(w, seed=sd, shot=sh, steps=nst)
+            mixed = rq.results[pos]
             with F.Plan(s, gpu_device) as plan:
-                plan.run(steps)
+                plan.run(nst)
                 plan.sync()
                 alone, _ = plan.read_detectors()
-            mixed = rq.results[pos]
             assert np.array_equal(alone.view(np.uint32), mixed.view(np.uint32)), (rq.rid, pos)
+            ref = Oracle(s, 32, threads=8)
+            ref.run(nst)
+            want = ref.read_detectors()[0].astype(np.float32)
+            ref.close()
+            assert np.array_equal(mixed.view(np.uint32), want.view(np.uint32)), ("oracle", rq.rid, pos)
+            worst = max(worst, fp64_check(s, mixed, f"served request {rq.rid} shot {pos}", threads=8))
+    print(f"served shots: worst detector rel L2 vs FP64 oracle {worst:.3e}", flush=True)
