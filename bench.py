@@ -123,14 +123,28 @@ def _pick(d: dict, pred) -> tuple:
     return k, d[k]
 
 
+def _step_kernel_mode(name: str) -> int:
+    """Last template argument of a single-grid step kernel's name
+    (step_kernel<T, C, D, medium, mode>), -1 for other kernels."""
+    if "step_kernel<" not in name:
+        return -1
+    args = name.split("step_kernel<", 1)[1].split(">", 1)[0].split(",")
+    try:
+        return int(args[-1])
+    except ValueError:
+        return -1
+
+
 def pmc_summary(paths: list, copy_mib: int, traffic_file: str) -> int:
     fetch = _read_counter(paths[0], "FETCH_SIZE")
     write = _read_counter(paths[1], "WRITE_SIZE")
     cfetch = _read_counter(paths[2], "FETCH_SIZE")
     cwrite = _read_counter(paths[3], "WRITE_SIZE")
-    interior = lambda n: "step_kernel" in n and "false" in n
-    kname, fkib = _pick(fetch, interior)
-    _, wkib = _pick(write, interior)
+    # the single-grid kernel that covers the interior: the fused one (mode 2),
+    # or the interior one of a two-kernel launch (mode 0), not the CPML-class
+    # kernel beside it (mode 1)
+    kname, fkib = _pick(fetch, lambda n: _step_kernel_mode(n) in (0, 2))
+    wkib = write[kname]
     cname, cf = _pick(cfetch, lambda n: "copy" in n.lower() or "elementwise" in n.lower())
     _, cw = _pick(cwrite, lambda n: "copy" in n.lower() or "elementwise" in n.lower())
     copy_bytes = copy_mib * (1 << 20)
@@ -381,9 +395,13 @@ def main() -> int:
     elapsed = time.perf_counter() - t_start
     avg_ms, nlaunch = plan.kernel_ms(0)
     cpml_ms, _ = plan.kernel_ms(1)
-    cells_int = plan.info("cells_interior")
+    fused = int(plan.info("fused"))
     cells_cpml = plan.info("cells_cpml")
-    plan_c = plan.info("cols_per_lane")
+    # the timed kernel: the fused one covers every cell, else the interior one
+    cells_int = plan.info("cells_interior") + (cells_cpml if fused else 0)
+    plan_c = int(plan.info("cols_per_lane"))
+    plan_d = int(plan.info("prefetch"))
+    kernel_sig = f"step_kernel<{T}, {plan_c}, {plan_d}, 1, {2 if fused else 0}>"
     det, _ = plan.read_detectors()
     plan.close()
     # only the rows of the steps run so far can hold samples
@@ -391,9 +409,9 @@ def main() -> int:
     cells = nx * ny
     value = cells * args.steps / elapsed / 1e9
     ms_per_step = elapsed * 1e3 / args.steps
-    # The timed kernel is the interior one. The run splits --steps into
-    # launches of min(T, steps left); each launch moves the state of its
-    # cells through HBM once, whatever its step count.
+    # The run splits --steps into launches of min(T, steps left); each
+    # launch moves the state of its cells through HBM once, whatever its
+    # step count.
     kernel_s = avg_ms * nlaunch * 1e-3
     launch_bytes = BYTES_PER_CELL_UPDATE * cells_int
     achieved = launch_bytes * nlaunch / kernel_s / 1e9 if kernel_s > 0 else 0.0
@@ -403,13 +421,13 @@ def main() -> int:
     if tf.exists():
         try:
             rec = json.loads(tf.read_text())
-            # only a record taken on this launch shape describes this kernel
-            if f"step_kernel<{T}, {int(plan_c)}, " in rec.get("kernel", ""):
+            # only a record taken on this kernel describes it
+            if kernel_sig in rec.get("kernel", ""):
                 traffic = rec.get("bytes_per_launch")
         except Exception:
             traffic = None
-    log(f"GPU: {value:.1f} Gcell/s, {ms_per_step:.3f} ms/step, interior kernel avg {avg_ms:.4f} ms over {nlaunch} "
-        f"launches (T={T}, {cells_int / cells:.3f} of the grid): {achieved:.0f} GB/s of one pass per launch "
+    log(f"GPU: {value:.1f} Gcell/s, {ms_per_step:.3f} ms/step, {kernel_sig} avg {avg_ms:.4f} ms over {nlaunch} "
+        f"launches ({cells_int / cells:.3f} of the grid): {achieved:.0f} GB/s of one pass per launch "
         f"(frac {achieved / HBM_PEAK_GBS:.3f}), single-step equivalent {step_equiv:.0f} GB/s; traces {check_msg}")
 
     cpu = None
@@ -472,15 +490,17 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_frac": None if traffic is None else round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "kernel": f"interior step kernel (T={T}, {int(plan_c)} columns per lane)",
+            "kernel": f"{'fused' if fused else 'interior'} step kernel {kernel_sig}",
             "kernel_avg_ms": round(avg_ms, 4),
             "launches": nlaunch,
             "cells_per_launch": int(cells_int),
             "bytes_per_launch": int(launch_bytes),
             "step_equiv_gbs": round(step_equiv, 1),
-            "cpml_kernel": {"avg_ms": round(cpml_ms, 4), "cells_per_launch": int(cells_cpml),
-                            "gbs_one_pass": round(BYTES_PER_CELL_UPDATE * cells_cpml / (cpml_ms * 1e-3) / 1e9, 1)
-                            if cpml_ms > 0 else None},
+            "cpml_cells_per_launch": int(cells_cpml),
+            "cpml_kernel": None if fused else {
+                "avg_ms": round(cpml_ms, 4),
+                "gbs_one_pass": round(BYTES_PER_CELL_UPDATE * cells_cpml / (cpml_ms * 1e-3) / 1e9, 1)
+                if cpml_ms > 0 else None},
         },
         "cpu_baseline": cpu,
     }

@@ -230,7 +230,9 @@ def tile_lists(nx: int, ny: int, npml: int, tblock: int = 0, cols_per_lane: int 
     """(entries, geometry): the tile partition a plan of this shape launches.
 
     events are (i, j) cells of sources, detectors and probes. entries is a
-    list of (first column, end column, first row, end row, class), geometry
+    list of (first column, end column, first row, end row, class), class 0
+    interior, 1 general CPML-class, 2 x-strip, 3 y-strip, 4 corner, 5 event
+    tiles; geometry
     the dict {vw, halo, band, band_cpml} of the interior tiles. Host-only;
     needs no GPU."""
     n = C.c_size_t()

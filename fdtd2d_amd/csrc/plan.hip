@@ -161,21 +161,40 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
   t.vw = 64 * C - 2 * t.halo;
   t.halo_e = T;
   t.vw_e = 64 - 2 * t.halo_e;
-  if (C == 1) {
-    t.x0 = 0;
-    t.ntx = (nx + t.vw - 1) / t.vw;
-  } else {
-    const int Pe = std::max(P, 1);
-    // the first multiple of vw_e at least Pe + halo + 2: a tile from x0 (or
-    // ending x0 before nx) loads no column of the strips or the ring
-    t.x0 = t.vw_e * ((Pe + t.halo + 1 + t.vw_e) / t.vw_e);
-    t.ntx = std::max(0, (nx - 2 * t.x0) / t.vw);
-  }
-  const int xr = t.x0 + t.ntx * t.vw;  // right margin [xr, nx) (C > 1)
+  const int Pe = std::max(P, 1);
+  // A CPML-class piece at the grid's left or right edge needs no halo on
+  // that side (the edge cells depend on nothing beyond the grid): it stores
+  // up to we columns of the 64 it loads.
+  const int we = 64 - t.halo_e;
+  // Interior tile tx stores [x0 + tx*vw, x0 + (tx+1)*vw); its window (halo
+  // columns on each side) must load no column of the x strips or the ring:
+  // x0 - halo > Pe, and the last tile's window ends before nx - Pe - 1. The
+  // margins [0, x0) and [xr, nx) go to CPML-class pieces. x0 is a multiple
+  // of 16 floats, so that the tiles' stored rows start on 64-byte
+  // boundaries (vw is a multiple of 16 at 8 steps per launch): at 8192^2 the
+  // interior kernel alone measured 0.667 ms per launch with x0 = 56 (rows
+  // starting 32 bytes into a 64-byte segment) against 0.608 with x0 = 48.
+  t.x0 = (Pe + t.halo + 1 + 15) / 16 * 16;
+  t.ntx = std::max(0, (nx - Pe - 2 - t.halo - t.x0) / t.vw);
+  const int xr = t.ntx > 0 ? t.x0 + t.ntx * t.vw : 0;  // right margin [xr, nx)
   t.H = band > 0 ? std::max(band, T) : pick_band(nx, ny, t.vw, T, C);
   t.nty = (ny + t.H - 1) / t.H;
   t.hc = band_cpml > 0 ? std::max(band_cpml, T) : std::max(32, T);
-  // window of rows / columns a tile loads
+  auto count = [](const int *pref, int a, int b) { return pref ? pref[b] - pref[a] : 0; };
+  // an event cell (source, detector, component detector) in rows [jl0, jl1), columns [cs, ce)
+  auto events_in = [&](int jl0, int jl1, int cs, int ce) {
+    if (!m) return false;
+    size_t lo = 0, hi = m->nev;  // first event at or after row jl0
+    while (lo < hi) {
+      const size_t mid = (lo + hi) / 2;
+      if (m->ev[mid].y < jl0) lo = mid + 1;
+      else hi = mid;
+    }
+    for (size_t q = lo; q < m->nev && m->ev[q].y < jl1; ++q)
+      if (m->ev[q].x >= cs && m->ev[q].x < ce) return true;
+    return false;
+  };
+  // window of rows / columns an interior tile loads
   auto window = [&](int tx, int r0, int r1, int &jl0, int &jl1, int &cs, int &ce) {
     jl0 = std::max(0, r0 - T);
     jl1 = std::min(ny, r1 + T);
@@ -187,19 +206,116 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
     if (!m) return false;
     int jl0, jl1, cs, ce;
     window(tx, r0, r1, jl0, jl1, cs, ce);
-    if (m->srow_pref && (m->srow_pref[jl1] != m->srow_pref[jl0] || m->scol_pref[ce] != m->scol_pref[cs]))
+    if (count(m->srow_pref, jl0, jl1) || count(m->lrow_pref, jl0, jl1) || count(m->scol_pref, cs, ce) ||
+        count(m->pcol_pref, cs, ce))
       return true;
-    // first event at or after row jl0
-    size_t lo = 0, hi = m->nev;
-    while (lo < hi) {
-      const size_t mid = (lo + hi) / 2;
-      if (m->ev[mid].y < jl0) lo = mid + 1;
-      else hi = mid;
-    }
-    for (size_t q = lo; q < m->nev && m->ev[q].y < jl1; ++q)
-      if (m->ev[q].x >= cs && m->ev[q].x < ce) return true;
-    return false;
+    return events_in(jl0, jl1, cs, ce);
   };
+  // Kind of a CPML-class piece storing columns [c0, c1) of rows [r0, r1),
+  // from the window it loads (see kKindGeneral .. kKindE), as flag bits.
+  auto piece_kind = [&](int c0, int r0, int r1) {
+    const int cs = std::max(0, c0 - t.halo_e), ce = std::min(nx, cs + 64);
+    const int jl0 = std::max(0, r0 - T), jl1 = std::min(ny, r1 + T);
+    const bool ev =
+        m && (events_in(jl0, jl1, cs, ce) || count(m->lrow_pref, jl0, jl1) || count(m->pcol_pref, cs, ce));
+    const bool rows = jl0 > Pe && jl1 < ny - Pe - 1 && !(m && count(m->srow_pref, jl0, jl1));
+    const bool cols = cs > Pe && ce < nx - Pe - 1 && !(m && count(m->scol_pref, cs, ce));
+    int kind = fdtd2d::kKindGeneral;
+    if (!ev) kind = rows ? fdtd2d::kKindX : cols ? fdtd2d::kKindY : fdtd2d::kKindXY;
+    else if (rows && cols) kind = fdtd2d::kKindE;
+    return kind << fdtd2d::kKindShift;
+  };
+  const int kX_flags = fdtd2d::kKindX << fdtd2d::kKindShift;
+  std::vector<int4> *lists = t.lists;
+  auto add_piece = [&](int c0, int c1, int r0, int r1, int kind) {
+    lists[1].push_back(make_int4(edge_entry_x(c0, c1 - c0), r0, r1, 1 | kind));
+  };
+  // rows [ytop, ybot) are where a band's window (T rows of cone above and
+  // below) can stay clear of the y strips and the ring
+  const int ytop = Pe + T + 1, ybot = ny - Pe - T - 2;
+  // Sub-bands of band [jb0, jb1): hc rows, ending at the first row whose
+  // window clears the top strip and at the last one clear of the bottom
+  // strip, so the rows between them and the strips can run as longer tiles
+  // (when the band keeps at least a sub-band's height of them: shorter ones
+  // cost more in cone rows than they save).
+  auto sub_bands = [&](int jb0, int jb1, std::vector<int2> &out) {
+    out.clear();
+    for (int r0 = jb0, r1; r0 < jb1; r0 = r1) {
+      r1 = std::min(jb1, r0 + t.hc);
+      if (r0 < ytop && r1 > ytop && jb1 - ytop >= t.hc) r1 = ytop;
+      if (r0 < ybot && r1 > ybot && ybot - jb0 >= t.hc) r1 = ybot;
+      out.push_back(make_int2(r0, r1));
+    }
+  };
+  std::vector<int2> subs;
+  for (int tx = 0; tx < t.ntx; ++tx)
+    for (int ty = 0; ty < t.nty; ++ty) {
+      const int jb0 = ty * t.H, jb1 = std::min(ny, jb0 + t.H);
+      if (!touches(tx, jb0, jb1)) {
+        lists[0].push_back(make_int4(tx, jb0, jb1, 0));
+        continue;
+      }
+      sub_bands(jb0, jb1, subs);
+      int run0 = -1;  // start of the current run of interior sub-bands
+      for (const int2 &sb : subs) {
+        const int r0 = sb.x, r1 = sb.y;
+        if (touches(tx, r0, r1)) {
+          if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, r0, 0));
+          run0 = -1;
+          const int c1 = std::min(nx, t.x0 + tx * t.vw + t.vw);
+          for (int c0 = t.x0 + tx * t.vw; c0 < c1; c0 += t.vw_e) {
+            const int e1 = std::min(c0 + t.vw_e, c1);
+            add_piece(c0, e1, r0, r1, piece_kind(c0, r0, r1));
+          }
+        } else if (run0 < 0) {
+          run0 = r0;
+        }
+      }
+      if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, jb1, 0));
+    }
+  // The margins: pieces of one column per lane across [0, x0) and [xr, nx)
+  // (all of [0, nx) when no interior tile fits), on whole bands where the
+  // band's window makes them x-strip pieces, else on sub-bands, each of its
+  // own kind, with runs of x-strip sub-bands merged.
+  std::vector<int2> cols;
+  auto margin_cols = [&](int a, int b) {
+    for (int c = a; c < b;) {
+      int w;
+      if (c == 0) w = (b == nx && nx <= 64) ? nx : std::min(b, we);
+      else if (b == nx && c >= nx - we) w = nx - c;
+      else if (b == nx) w = std::min(t.vw_e, nx - we - c);
+      else w = std::min(t.vw_e, b - c);
+      cols.push_back(make_int2(c, c + w));
+      c += w;
+    }
+  };
+  if (t.ntx > 0) {
+    margin_cols(0, t.x0);
+    margin_cols(xr, nx);
+  } else {
+    margin_cols(0, nx);
+  }
+  for (const int2 &pc : cols)
+    for (int ty = 0; ty < t.nty; ++ty) {
+      const int jb0 = ty * t.H, jb1 = std::min(ny, jb0 + t.H);
+      if (piece_kind(pc.x, jb0, jb1) == kX_flags) {
+        add_piece(pc.x, pc.y, jb0, jb1, kX_flags);
+        continue;
+      }
+      sub_bands(jb0, jb1, subs);
+      int run0 = -1;  // start of the current run of x-strip sub-bands
+      for (const int2 &sb : subs) {
+        const int kind = piece_kind(pc.x, sb.x, sb.y);
+        if (kind == kX_flags) {
+          if (run0 < 0) run0 = sb.x;
+          continue;
+        }
+        if (run0 >= 0) add_piece(pc.x, pc.y, run0, sb.x, kX_flags);
+        run0 = -1;
+        add_piece(pc.x, pc.y, sb.x, sb.y, kind);
+      }
+      if (run0 >= 0) add_piece(pc.x, pc.y, run0, jb1, kX_flags);
+    }
   auto needs_jz = [&](const int4 &w) {
     if (!m || !m->ne_lo) return 1;
     int jl0, jl1, cs, ce;
@@ -224,50 +340,6 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
     }
     return true;
   };
-  std::vector<int4> *lists = t.lists;
-  // rows [ytop, ybot) are where a band's window (T rows of cone above and
-  // below) can stay clear of the y strips and the ring
-  const int Pe = std::max(P, 1);
-  const int ytop = Pe + T + 1, ybot = ny - Pe - T - 2;
-  for (int tx = 0; tx < t.ntx; ++tx)
-    for (int ty = 0; ty < t.nty; ++ty) {
-      const int jb0 = ty * t.H, jb1 = std::min(ny, jb0 + t.H);
-      if (!touches(tx, jb0, jb1)) {
-        lists[0].push_back(make_int4(tx, jb0, jb1, 0));
-        continue;
-      }
-      int run0 = -1;  // start of the current run of interior sub-bands
-      for (int r0 = jb0, r1; r0 < jb1; r0 = r1) {
-        r1 = std::min(jb1, r0 + t.hc);
-        // sub-bands end at the first row whose window clears the top strip
-        // and at the last one clear of the bottom strip, so the rows between
-        // them and the strips can run as interior tiles (when the band keeps
-        // at least a sub-band's height of them: shorter interior tiles cost
-        // more in cone rows than they save)
-        if (r0 < ytop && r1 > ytop && jb1 - ytop >= t.hc) r1 = ytop;
-        if (r0 < ybot && r1 > ybot && ybot - jb0 >= t.hc) r1 = ybot;
-        if (touches(tx, r0, r1)) {
-          if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, r0, 0));
-          run0 = -1;
-          const int c1 = std::min(nx, t.x0 + tx * t.vw + t.vw);
-          for (int c0 = t.x0 + tx * t.vw; c0 < c1; c0 += t.vw_e)
-            lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, c1 - c0)), r0, r1, 0));
-        } else if (run0 < 0) {
-          run0 = r0;
-        }
-      }
-      if (run0 >= 0) lists[0].push_back(make_int4(tx, run0, jb1, 0));
-    }
-  // the margins left and right of the interior tiles (C > 1): CPML-class
-  // pieces on sub-bands of hc rows
-  if (C > 1)
-    for (int r0 = 0; r0 < ny; r0 += t.hc) {
-      const int r1 = std::min(ny, r0 + t.hc);
-      for (int c0 = 0; c0 < std::min(t.x0, nx); c0 += t.vw_e)
-        lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, std::min(t.x0, nx) - c0)), r0, r1, 0));
-      for (int c0 = std::max(xr, t.x0); c0 < nx; c0 += t.vw_e)
-        lists[1].push_back(make_int4(edge_entry_x(c0, std::min(t.vw_e, nx - c0)), r0, r1, 0));
-    }
   // Order by column group of 4 tiles, then first row, then tile: the 4 waves
   // of a work-group get neighbouring tiles and consecutive work-groups go down
   // the same columns. Pad each list to whole work-groups with empty entries.
@@ -283,11 +355,15 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
       return u.x < v.x;
     });
     for (int4 &w : lists[c]) {
-      w.w = c == 0 ? needs_jz(w) : 1;
-      if (c == 0 && w.w && plasma_full(w)) w.w |= 2;
+      if (c == 0) {
+        w.w = needs_jz(w);
+        if (w.w && plasma_full(w)) w.w |= 2;
+      }
       const long long c0 = c == 0 ? t.x0 + static_cast<long long>(w.x) * t.vw : edge_entry_c0(w.x);
       const long long c1 = c == 0 ? std::min<long long>(nx, c0 + t.vw) : c0 + edge_entry_w(w.x);
-      t.cells[c] += static_cast<double>(std::max<long long>(0, c1 - c0)) * (w.z - w.y);
+      const double cells = static_cast<double>(std::max<long long>(0, c1 - c0)) * (w.z - w.y);
+      t.cells[c] += cells;
+      if (c == 1) t.cells_kind[fdtd2d::entry_kind(w.w)] += cells;
     }
     t.n_tiles[c] = static_cast<int>(lists[c].size());
     if (c == 0)
@@ -303,36 +379,73 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
 // The step kernel hands list positions [x*q + min(x, r), ...) of a G
 // work-group launch (q = G/8, r = G%8) to XCD x in launch order. Work-groups
 // with a Jz update take about 1.6 times as long as those without; moving
-// them to the front of each XCD's range (keeping the order within each kind)
+// them to the front of an XCD's range (keeping the order within each kind)
 // leaves the short ones for the end of the launch, where they fill the
-// compute units that finish early.
-static void order_jz_first(std::vector<int4> &list) {
+// compute units that finish early. Reorders work-groups [g0, g1).
+static void jz_first_range(std::vector<int4> &list, int g0, int g1) {
   const int W = fdtd2d::kWavesPerWG;
-  const int G = static_cast<int>(list.size()) / W, q = G >> 3, r = G & 7;
   std::vector<int4> jz, rest;
-  for (int x = 0; x < 8; ++x) {
-    const int g0 = x * q + std::min(x, r), g1 = g0 + q + (x < r ? 1 : 0);
-    jz.clear();
-    rest.clear();
-    for (int g = g0; g < g1; ++g) {
-      bool any = false;
-      for (int w = 0; w < W; ++w) {
-        const int4 &e = list[static_cast<size_t>(g) * W + w];
-        any = any || (e.y < e.z && (e.w & 1));
-      }
-      std::vector<int4> &d = any ? jz : rest;
-      d.insert(d.end(), list.begin() + static_cast<size_t>(g) * W, list.begin() + static_cast<size_t>(g + 1) * W);
+  for (int g = g0; g < g1; ++g) {
+    bool any = false;
+    for (int w = 0; w < W; ++w) {
+      const int4 &e = list[static_cast<size_t>(g) * W + w];
+      any = any || (e.y < e.z && (e.w & 1));
     }
-    std::copy(jz.begin(), jz.end(), list.begin() + static_cast<size_t>(g0) * W);
-    std::copy(rest.begin(), rest.end(), list.begin() + static_cast<size_t>(g0) * W + jz.size());
+    std::vector<int4> &d = any ? jz : rest;
+    d.insert(d.end(), list.begin() + static_cast<size_t>(g) * W, list.begin() + static_cast<size_t>(g + 1) * W);
+  }
+  std::copy(jz.begin(), jz.end(), list.begin() + static_cast<size_t>(g0) * W);
+  std::copy(rest.begin(), rest.end(), list.begin() + static_cast<size_t>(g0) * W + jz.size());
+}
+
+static void order_jz_first(std::vector<int4> &list) {
+  const int G = static_cast<int>(list.size()) / fdtd2d::kWavesPerWG, q = G >> 3, r = G & 7;
+  for (int x = 0; x < 8; ++x) {
+    const int g0 = x * q + std::min(x, r);
+    jz_first_range(list, g0, g0 + q + (x < r ? 1 : 0));
   }
 }
+
+// One list for the fused launch: XCD x's range of the G = Gc + Gi
+// work-groups (q + (x < r) of them, q = G/8, r = G%8) starts with its share
+// of the x-strip and y-strip work-groups (Gc/8, one more for x < Gc%8; never
+// more than the range holds), contiguous in their column order and flagged
+// kFlagCpml, and continues with the next interior work-groups in column
+// order (those with a Jz update first when jz_first is set). The strip
+// work-groups, the longer ones per cell, start at the launch's beginning on
+// every XCD.
+static std::vector<int4> fused_list(const std::vector<int4> &interior, std::vector<int4> cpml, bool jz_first) {
+  const int W = fdtd2d::kWavesPerWG;
+  const int Gi = static_cast<int>(interior.size()) / W, Gc = static_cast<int>(cpml.size()) / W, G = Gi + Gc;
+  for (int4 &e : cpml) e.w |= fdtd2d::kFlagCpml;
+  std::vector<int4> out;
+  out.reserve(static_cast<size_t>(G) * W);
+  const int q = G >> 3, r = G & 7;
+  size_t ic = 0, ii = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int sx = q + (x < r ? 1 : 0);
+    const int cx = std::min(sx, Gc / 8 + (x < Gc % 8 ? 1 : 0));
+    const size_t nc = static_cast<size_t>(cx) * W, ni = static_cast<size_t>(sx - cx) * W;
+    out.insert(out.end(), cpml.begin() + ic, cpml.begin() + ic + nc);
+    const int g_int = static_cast<int>(out.size()) / W;
+    out.insert(out.end(), interior.begin() + ii, interior.begin() + ii + ni);
+    if (jz_first) jz_first_range(out, g_int, g_int + sx - cx);
+    ic += nc;
+    ii += ni;
+  }
+  return out;
+}
+
+// The fused launch is compiled for one and two columns per lane.
+static bool fused_launch(const fdtd2d_plan *pl) { return pl->fused && pl->C <= 2; }
 
 int set_tiling(fdtd2d_plan *pl) {
   const int T = pl->T > 0 ? pl->T : kDefaultT;
   TileMasks m;
   m.srow_pref = pl->srow_pref.data();
   m.scol_pref = pl->scol_pref.data();
+  m.lrow_pref = pl->lrow_pref.data();
+  m.pcol_pref = pl->pcol_pref.data();
   m.ne_lo = pl->ne_lo.data();
   m.ne_hi = pl->ne_hi.data();
   m.jz_lo = pl->jz_lo.data();
@@ -359,6 +472,7 @@ int set_tiling(fdtd2d_plan *pl) {
     pl->cells_class[c] = t.cells[c];
     a.n_tiles[c] = t.n_tiles[c];
   }
+  for (int k = 0; k < 5; ++k) pl->cells_kind[k] = t.cells_kind[k];
   pl->n_nojz = t.n_nojz;
   pl->n_full = t.n_full;
   if (pl->pooled) {  // the pool concatenates the host lists of its shots
@@ -368,10 +482,32 @@ int set_tiling(fdtd2d_plan *pl) {
     pl->tiling = std::move(t);
     return FDTD2D_OK;
   }
-  if (pl->jz_first) order_jz_first(t.lists[0]);
-  const std::vector<int4> *lists = t.lists;
+  std::vector<int4> lists[2];
+  if (fused_launch(pl)) {
+    // the entries of the kinds the fused kernel carries join the interior
+    // ones (event entries only with no captures); the rest keep the
+    // CPML-class kernel
+    const bool captures = pl->a.nprobe > 0 || pl->line_rows > 0;
+    const int kinds = fdtd2d::fused_kinds(pl->a.medium_kind);
+    std::vector<int4> xy;
+    for (const int4 &e : t.lists[1]) {
+      if (e.y >= e.z) continue;  // padding
+      const int k = fdtd2d::entry_kind(e.w);
+      const bool in = ((kinds >> k) & 1) && !(k == fdtd2d::kKindE && captures);
+      (in ? xy : lists[1]).push_back(e);
+    }
+    for (std::vector<int4> *v : {&xy, &lists[1]})
+      while (v->size() % fdtd2d::kWavesPerWG) v->push_back(make_int4(0, 0, 0, 0));
+    lists[0] = fused_list(t.lists[0], std::move(xy), pl->jz_first != 0);
+  } else {
+    lists[0] = std::move(t.lists[0]);
+    lists[1] = std::move(t.lists[1]);
+    if (pl->jz_first) order_jz_first(lists[0]);
+  }
   const size_t n = lists[0].size() + lists[1].size();
+  // the launches queued so far read the current lists
   if (pl->wg_buf) {
+    HIP_TRY(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
     hipFree(pl->wg_buf);
     pl->wg_buf = nullptr;
   }
@@ -479,7 +615,7 @@ int fdtd2d_tile_lists(int nx, int ny, int npml, int tblock, int cols_per_lane, i
       e[1] = c1;
       e[2] = w.y;
       e[3] = w.z;
-      e[4] = c;
+      e[4] = c == 0 ? 0 : 1 + fdtd2d::entry_kind(w.w);
     }
   return FDTD2D_OK;
 }
@@ -700,20 +836,23 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     u_hy = p->den_hy[jm];
     u_ey = p->den_ey[jm];
     auto same = [](float x, float y) { return std::memcmp(&x, &y, 4) == 0; };
-    // RC probe columns are special too: only the CPML-class kernel captures them
+    // RC probe columns and the processing line's row are special too: only
+    // general CPML-class tiles capture them
     std::vector<char> probe(nx, 0);
     for (int e = 0; e < p->nrc; ++e) probe[p->rc_i[e]] = probe[p->rc_imax[e]] = 1;
     pl->scol_pref.assign(nx + 1, 0);
+    pl->pcol_pref.assign(nx + 1, 0);
     for (int i = 0; i < nx; ++i) {
-      const bool sp = i >= nx - 1 || !same(p->den_hx[i], u_hx) || !same(p->den_ex[i], u_ex) || probe[i];
+      const bool sp = i >= nx - 1 || !same(p->den_hx[i], u_hx) || !same(p->den_ex[i], u_ex);
       pl->scol_pref[i + 1] = pl->scol_pref[i] + sp;
+      pl->pcol_pref[i + 1] = pl->pcol_pref[i] + (probe[i] != 0);
     }
     pl->srow_pref.assign(ny + 1, 0);
-    // so is the row of the processing line
+    pl->lrow_pref.assign(ny + 1, 0);
     for (int j = 0; j < ny; ++j) {
-      const bool sp = j >= ny - 1 || !same(p->den_hy[j], u_hy) || !same(p->den_ey[j], u_ey) ||
-                      (L.line_rows > 0 && j == p->line_j);
+      const bool sp = j >= ny - 1 || !same(p->den_hy[j], u_hy) || !same(p->den_ey[j], u_ey);
       pl->srow_pref[j + 1] = pl->srow_pref[j] + sp;
+      pl->lrow_pref[j + 1] = pl->lrow_pref[j] + (L.line_rows > 0 && j == p->line_j);
     }
   }
   {
@@ -899,6 +1038,9 @@ int fdtd2d_plan_set_option(fdtd2d_plan *pl, const char *name, double value) {
   } else if (n == "jz_first") {
     if (v != 0 && v != 1) return set_error(FDTD2D_EINVAL, "jz_first must be 0 or 1");
     pl->jz_first = v;
+  } else if (n == "fused") {
+    if (v != 0 && v != 1) return set_error(FDTD2D_EINVAL, "fused must be 0 or 1");
+    pl->fused = v;
   } else if (n == "prefetch") {
     const int maxd = pl->C == 1 ? 4 : pl->C == 2 ? 2 : 1;
     if (v != 1 && v != 2 && v != 4) return set_error(FDTD2D_EINVAL, "prefetch must be 1, 2 or 4");
@@ -927,12 +1069,18 @@ int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value)
   else if (n == "tiles_plasma_full") *value = pl->n_full;
   else if (n == "cells_interior") *value = pl->cells_class[0];
   else if (n == "cells_cpml") *value = pl->cells_class[1];
+  else if (n == "cells_cpml_general") *value = pl->cells_kind[fdtd2d::kKindGeneral];
+  else if (n == "cells_xstrip") *value = pl->cells_kind[fdtd2d::kKindX];
+  else if (n == "cells_ystrip") *value = pl->cells_kind[fdtd2d::kKindY];
+  else if (n == "cells_corner") *value = pl->cells_kind[fdtd2d::kKindXY];
+  else if (n == "cells_events") *value = pl->cells_kind[fdtd2d::kKindE];
   else if (n == "tile_cols") *value = pl->a.vw;
   else if (n == "bytes") *value = static_cast<double>(pl->L.total);
   else if (n == "steps") *value = pl->n;
   else if (n == "cols_per_lane") *value = pl->C;
   else if (n == "prefetch") *value = pl->D;
   else if (n == "jz_first") *value = pl->jz_first;
+  else if (n == "fused") *value = fused_launch(pl) ? 1 : 0;
   else return set_error(FDTD2D_EINVAL, "unknown info name " + n);
   return FDTD2D_OK;
 }
@@ -973,7 +1121,8 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
     }
     // timing events bracket each kernel alone on its own stream
     if (pl->timing) HIP_TRY(hipEventRecord(e0, pl->stream), "hipEventRecord");
-    hipError_t e = fdtd2d::launch_step(pl->a, 0, t, pl->C, pl->D, pl->src, pl->n, pl->stream);
+    hipError_t e = fused_launch(pl) ? fdtd2d::launch_step_fused(pl->a, t, pl->C, pl->D, pl->src, pl->n, pl->stream)
+                             : fdtd2d::launch_step(pl->a, 0, t, pl->C, pl->D, pl->src, pl->n, pl->stream);
     if (e != hipSuccess) return hip_fail(e, "step kernel launch");
     if (pl->timing) {
       HIP_TRY(hipEventRecord(e1, pl->stream), "hipEventRecord");

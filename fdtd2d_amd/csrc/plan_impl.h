@@ -70,10 +70,15 @@ struct Tiling {
   int n_nojz = 0;  // interior tiles that skip the Jz update
   int n_full = 0;  // interior tiles whose window is full of plasma
   double cells[2] = {0, 0};
+  double cells_kind[5] = {0, 0, 0, 0, 0};  // CPML-class cells by kind (kKindGeneral ... kKindE)
 };
 
 struct TileMasks {
-  const int *srow_pref = nullptr, *scol_pref = nullptr;  // [ny+1], [nx+1]
+  // prefix counts of the rows / columns whose den differ from the uniform
+  // interior values (and the last row / column): [ny+1], [nx+1]
+  const int *srow_pref = nullptr, *scol_pref = nullptr;
+  // prefix counts of the processing line's row and of the RC probe columns
+  const int *lrow_pref = nullptr, *pcol_pref = nullptr;
   const int *ne_lo = nullptr, *ne_hi = nullptr, *jz_lo = nullptr, *jz_hi = nullptr;  // [ny]
   // per row, the runs [x, y) of consecutive columns with Ne > 0: row r's
   // runs are ne_runs[ne_run_off[r] .. ne_run_off[r+1])
@@ -109,7 +114,9 @@ struct fdtd2d_plan {
   int D = 2;     // rows of prefetch per wave (interior kernel); the row loop is unrolled by D
   int band_cpml = 0;               // rows per CPML-class work-group (0: automatic)
   int jz_first = 1;  // interior work-groups with a Jz update first in each XCD's range
+  int fused = 1;     // one kernel per launch over one list (0: interior and CPML-class kernels on two streams)
   double cells_class[2] = {0, 0};  // grid cells owned by interior / CPML-class tiles
+  double cells_kind[5] = {0, 0, 0, 0, 0};  // CPML-class cells by kind: general, x strip, y strip, corner, events
   int hc = 0;                      // rows per CPML-class sub-band in use
   int n_nojz = 0;                  // interior tiles skipping the Jz update
   int n_full = 0;                  // interior tiles whose window is full of plasma
@@ -131,6 +138,7 @@ struct fdtd2d_plan {
   // plasma (Ne > 0) or a nonzero Jz: windows outside every span skip the Jz
   // update (there Jz stays +0 and Ez - Jz = Ez).
   std::vector<int> srow_pref, scol_pref;
+  std::vector<int> lrow_pref, pcol_pref;  // the processing line's row, the RC probe columns
   std::vector<int> ne_lo, ne_hi, jz_lo, jz_hi;
   // runs of plasma columns per row (see TileMasks): an interior tile whose
   // whole window lies inside them skips the Ne > 0 test (flag bit 1)

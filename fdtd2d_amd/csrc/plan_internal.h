@@ -125,10 +125,36 @@ constexpr int kMaxT = 8;
 // (low kEdgeColBits bits) and the stored width (above).
 constexpr int kEdgeColBits = 20;
 constexpr int kWavesPerWG = 4;
+// Kind of a CPML-class entry (flags bits 2-4), from the window the entry
+// loads. Rows clear: no y-strip, ring or special-den row; columns clear: no
+// x-strip, ring or special-den column; events: a source, detector or
+// component cell, an RC probe column or the processing line's row.
+//   x strip: no events, rows clear         y strip: no events, columns clear
+//   corner:  no events                     events: rows and columns clear
+//   general: the rest (an event inside a strip or beside a special row)
+enum : int { kKindGeneral = 0, kKindX = 1, kKindY = 2, kKindXY = 3, kKindE = 4 };
+constexpr int kKindShift = 2;
+inline __host__ __device__ int entry_kind(int flags) { return (flags >> kKindShift) & 7; }
+// A CPML-class entry in the one list of a fused single-grid launch.
+constexpr int kFlagCpml = 32;
+// CPML-class kinds the fused single-grid kernel carries (bit 1 << kind) for
+// a medium (0 table, 1 density): those that keep it within 3 waves per SIMD
+// with no scratch spills at 8 steps per launch (with the corner kind the
+// table medium's kernel spills 46 VGPRs). Event entries of problems with RC
+// probes or a processing line never go there (their capturing row loop
+// would spill 180); the rest run in the CPML-class kernel beside it.
+constexpr int fused_kinds(int medium_kind) {
+  return medium_kind == 1 ? (1 << kKindX) | (1 << kKindY) | (1 << kKindXY) | (1 << kKindE)
+                          : (1 << kKindX) | (1 << kKindY) | (1 << kKindE);
+}
 
 // Launches the step kernel of work-group class `cls` (0 interior, 1 CPML)
 // for T steps on `stream`, D rows of prefetch; src = buffer set read.
 hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream);
+// Launches the fused single-grid kernel over the combined list (tile_list[0],
+// n_wg[0] work-groups: CPML-class entries flagged kFlagCpml first in each
+// XCD's range, then interior ones). C = 1 or 2.
+hipError_t launch_step_fused(const StepArgs &a, int T, int C, int D, int src, int n0, hipStream_t stream);
 
 // RC cutoff lines (Dev_RCcalc semantics) for steps n0+1 .. n0+t from the
 // capture of the launch that computed them; one work-group per line.

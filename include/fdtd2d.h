@@ -289,15 +289,26 @@ int fdtd2d_plan_set_tblock(fdtd2d_plan *plan, int steps_per_launch);
    that touch the CPML, 0 = automatic), "cols_per_lane" (1, 2 or 4), "prefetch"
    (rows loaded ahead per wave: 1, 2 or 4 with 1 column per lane, 1 or 2 with
    2, 1 with 4), "jz_first" (1, the default: interior work-groups that update
-   Jz are launched first on each XCD; 0: column order only). Results are
-   identical for every choice. */
+   Jz are launched first on each XCD; 0: column order only), "fused" (1, the
+   default: one kernel per launch runs the interior tiles and every
+   CPML-class tile but a few kinds - the general ones, and for the table
+   medium the corners - which run in a second kernel beside it only where
+   the grid has them; 0: all CPML-class tiles in the second kernel; with 4
+   columns per lane launches are never fused). Results are identical for
+   every choice. */
 int fdtd2d_plan_set_option(fdtd2d_plan *plan, const char *name, double value);
 /* Plan facts by name: "pitch", "tblock", "band", "band_cpml", "ntx", "nty",
-   "workgroups", "workgroups_cpml", "tile_cols", "bytes", "steps",
-   "cols_per_lane", "prefetch", "jz_first", "cells_interior", "cells_cpml",
+   "workgroups", "workgroups_cpml" (work-groups of the second kernel),
+   "tile_cols", "bytes", "steps", "cols_per_lane", "prefetch", "jz_first",
+   "fused" (1 when launches are fused), "cells_interior", "cells_cpml",
    "tiles_interior", "tiles_cpml", "tiles_nojz" (interior tiles with no Jz
    update), "tiles_plasma_full" (interior tiles whose window has Ne > 0 in
-   every cell). */
+   every cell), "cells_xstrip", "cells_ystrip", "cells_corner",
+   "cells_events", "cells_cpml_general" (the CPML-class cells by the kind of
+   tile that owns them: x-strip tiles carry only the x-strip psi and column
+   tables, y-strip tiles only the y-strip psi and row records, corner tiles
+   both, event tiles only the sources, detectors and captures, general tiles
+   all of these). */
 int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
 /* Advance nsteps steps (asynchronous on the plan's stream). */
 int fdtd2d_run(fdtd2d_plan *plan, int nsteps);
@@ -317,10 +328,12 @@ int fdtd2d_write_psi(fdtd2d_plan *plan, int which, const float *in, size_t count
    detectors, probes) at (ev_i[e], ev_j[e]): entries[5k..5k+4] = {first
    column, end column, first row, end row, class} of the cells wave k stores,
    for k < *count; class 0 interior kernel (cols_per_lane columns per lane),
-   1 CPML-class kernel (always one column per lane: a CPML-class tile of the
-   interior geometry is split into pieces of 64 - 2*tblock columns; with more
-   than one column per lane the interior tiles start after, and end before,
-   margins of such pieces as wide as the x-strip and the halo need).
+   1-5 CPML-class tiles (always one column per lane): 1 general, 2 x-strip,
+   3 y-strip, 4 corner, 5 event tiles. A CPML-class tile of the interior geometry is split into
+   pieces of 64 - 2*tblock columns; the interior tiles start after, and end
+   before, margins of such pieces as wide as the x-strip and the halo need,
+   whose pieces at the grid's edges store up to 64 - tblock columns (no halo
+   on the edge side).
    geometry (4 ints, may be null) = {vw, halo columns, rows per band, rows per
    CPML sub-band} of the interior tiles. entries may be null to query *count.
    tblock 0 = the default. No reference counterpart: the reference launches

@@ -13,8 +13,9 @@ sys.path.insert(0, str(ROOT))
 
 import bench  # noqa: E402
 
-INTERIOR = "void fdtd2d::(anonymous namespace)::step_kernel<8, 2, 2, 1, false>(fdtd2d::StepArgs, int, int)"
-EDGE = "void fdtd2d::(anonymous namespace)::step_kernel<8, 1, 1, 1, true>(fdtd2d::StepArgs, int, int)"
+INTERIOR = "void fdtd2d::(anonymous namespace)::step_kernel<8, 2, 2, 1, 0>(fdtd2d::StepArgs, int, int)"
+EDGE = "void fdtd2d::(anonymous namespace)::step_kernel<8, 1, 1, 1, 1>(fdtd2d::StepArgs, int, int)"
+FUSED = "void fdtd2d::(anonymous namespace)::step_kernel<8, 2, 2, 1, 2>(fdtd2d::StepArgs, int, int)"
 COPY = "__amd_rocclr_copyBuffer"
 
 
@@ -52,6 +53,18 @@ def test_pmc_summary_scales_by_the_copy_calibration(tmp_path, capsys):
     assert rec["write_bytes_per_launch"] == 50 * 1024
     assert rec["bytes_per_launch"] == 110 * 1024 * 2 + 50 * 1024
     assert "HBM traffic per launch" in capsys.readouterr().out
+
+
+def test_pmc_summary_takes_the_fused_kernel(tmp_path):
+    f, w, cf, cw = (tmp_path / n for n in ("f.csv", "w.csv", "cf.csv", "cw.csv"))
+    _counter_csv(f, "FETCH_SIZE", [(FUSED, 300.0), (COPY, 5.0)])
+    _counter_csv(w, "WRITE_SIZE", [(FUSED, 200.0)])
+    _counter_csv(cf, "FETCH_SIZE", [(COPY, 512.0)])
+    _counter_csv(cw, "WRITE_SIZE", [(COPY, 1024.0)])
+    out = tmp_path / "traffic.json"
+    assert bench.pmc_summary([str(f), str(w), str(cf), str(cw)], 1, str(out)) == 0
+    rec = json.loads(out.read_text())
+    assert rec["kernel"] == FUSED and rec["bytes_per_launch"] == 300 * 1024 * 2 + 200 * 1024
 
 
 def test_stats_summary_table(tmp_path, capsys):

@@ -86,6 +86,7 @@ def test_random_small_problems(gpu_device, seed):
         plan.set_option("prefetch", depth)
         plan.set_option("band_cpml", int(rng.choice([0, 1, 5, 9])))
         plan.set_option("jz_first", int(rng.integers(0, 2)))
+        plan.set_option("fused", int(rng.integers(0, 2)))
         plan.run(p.nsteps)
         plan.sync()
         g = all_state(plan, p.npml)
@@ -157,7 +158,7 @@ def test_interior_paths(gpu_device, seed):
     tblock = int(rng.integers(1, 9 if cols < 4 else 5))
     opts = {"cols_per_lane": cols, "tblock": tblock, "band": int(rng.choice([0, 16, 64])),
             "prefetch": int(rng.choice({1: [1, 2, 4], 2: [1, 2], 4: [1]}[cols])),
-            "band_cpml": int(rng.choice([0, 8, 32]))}
+            "band_cpml": int(rng.choice([0, 8, 32])), "fused": seed % 3 != 2}
     info = _run_pair(p, gpu_device, opts, f"interior seed {seed}")
     assert info["tiles_interior"] > 0
 
