@@ -162,6 +162,8 @@ SIGNATURES = [
     ("fdtd2d_processing_tof", C.c_int, [C.POINTER(ProcessingC), _f, C.c_int, _f, C.c_int, _f, _f, _f, _f, _f]),
     ("fdtd2d_write_processing_outputs", C.c_int, [C.POINTER(ProcessingC), C.c_char_p, _f, C.c_int]),
     ("fdtd2d_plan_create", C.c_int, [C.POINTER(ProblemC), C.c_int, C.POINTER(C.c_void_p)]),
+    ("fdtd2d_plan_create_in", C.c_int, [C.POINTER(ProblemC), C.c_int, C.c_void_p, C.c_size_t, C.c_void_p,
+                                        C.POINTER(C.c_void_p)]),
     ("fdtd2d_plan_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
     ("fdtd2d_plan_set_tblock", C.c_int, [C.c_void_p, C.c_int]),
     ("fdtd2d_plan_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),

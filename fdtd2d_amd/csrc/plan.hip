@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -122,6 +123,43 @@ namespace detail {
 
 int hip_fail(hipError_t e, const char *what) {
   return set_error(FDTD2D_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- static lists
+// The tile lists (the step kernels' work lists) of a plan made by
+// fdtd2d_plan_create_in live in this array instead of a hipMalloc of their
+// own: kStaticListEntries entries of 16 bytes (1 MiB) on every device the
+// library runs on, held until the process ends. Host code never passes the
+// array's name to a kernel or a copy (in host code it is a stand-in, not a
+// device address): it takes the device's own address from
+// hipGetSymbolAddress once per device, with that device current.
+constexpr size_t kStaticListEntries = 65536;
+static __device__ __align__(256) int4 scratch[kStaticListEntries];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_static_mu;
+static int4 *g_static_ptr[kMaxDevices];
+static bool g_static_busy[kMaxDevices];
+
+// The device address of `scratch` on `device` (which must be current).
+static int static_lists_ptr(int device, int4 **ptr) {
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (!g_static_ptr[device]) {
+    void *q = nullptr;
+    hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q) return hip_fail(e, "hipGetSymbolAddress of the static tile-list array");
+    g_static_ptr[device] = static_cast<int4 *>(q);
+  }
+  *ptr = g_static_ptr[device];
+  return FDTD2D_OK;
+}
+
+// Zeroes n 16-byte words (a kernel, not hipMemset: callers may capture the
+// stream the plan works on).
+__global__ void __launch_bounds__(256) zero_words(uint4 *p, size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 
@@ -439,6 +477,8 @@ static std::vector<int4> fused_list(const std::vector<int4> &interior, std::vect
 // The fused launch is compiled for one and two columns per lane.
 static bool fused_launch(const fdtd2d_plan *pl) { return pl->fused && pl->C <= 2; }
 
+// Builds the tile lists of the plan's launch shape and puts them where the
+// step kernels read them. Nothing of the plan changes unless this succeeds.
 int set_tiling(fdtd2d_plan *pl) {
   const int T = pl->T > 0 ? pl->T : kDefaultT;
   TileMasks m;
@@ -457,25 +497,29 @@ int set_tiling(fdtd2d_plan *pl) {
   m.ev = pl->ev_cells.data();
   m.nev = pl->ev_cells.size();
   Tiling t = make_tiling(pl->L.nx, pl->L.ny, pl->a.P, T, pl->C, pl->band, pl->band_cpml, &m);
+  // the geometry and counts the kernels and the info calls read
+  auto commit = [&](fdtd2d::StepArgs &a) {
+    a.halo = t.halo;
+    a.vw = t.vw;
+    a.halo_e = t.halo_e;
+    a.vw_e = t.vw_e;
+    a.x0 = t.x0;
+    a.trash_slots = std::min(kTrashSlots, kTrashRows * pl->L.pitch / (64 * pl->C));
+    a.ntx = t.ntx;
+    a.H = t.H;
+    a.nty = t.nty;
+    pl->hc = t.hc;
+    for (int c = 0; c < 2; ++c) {
+      pl->cells_class[c] = t.cells[c];
+      a.n_tiles[c] = t.n_tiles[c];
+    }
+    for (int k = 0; k < 5; ++k) pl->cells_kind[k] = t.cells_kind[k];
+    pl->n_nojz = t.n_nojz;
+    pl->n_full = t.n_full;
+  };
   fdtd2d::StepArgs &a = pl->a;
-  a.halo = t.halo;
-  a.vw = t.vw;
-  a.halo_e = t.halo_e;
-  a.vw_e = t.vw_e;
-  a.x0 = t.x0;
-  a.trash_slots = std::min(kTrashSlots, kTrashRows * pl->L.pitch / (64 * pl->C));
-  a.ntx = t.ntx;
-  a.H = t.H;
-  a.nty = t.nty;
-  pl->hc = t.hc;
-  for (int c = 0; c < 2; ++c) {
-    pl->cells_class[c] = t.cells[c];
-    a.n_tiles[c] = t.n_tiles[c];
-  }
-  for (int k = 0; k < 5; ++k) pl->cells_kind[k] = t.cells_kind[k];
-  pl->n_nojz = t.n_nojz;
-  pl->n_full = t.n_full;
   if (pl->pooled) {  // the pool concatenates the host lists of its shots
+    commit(a);
     a.n_wg[0] = static_cast<int>(t.lists[0].size()) / fdtd2d::kWavesPerWG;
     a.n_wg[1] = static_cast<int>(t.lists[1].size()) / fdtd2d::kWavesPerWG;
     a.tile_list[0] = a.tile_list[1] = nullptr;
@@ -505,21 +549,38 @@ int set_tiling(fdtd2d_plan *pl) {
     if (pl->jz_first) order_jz_first(lists[0]);
   }
   const size_t n = lists[0].size() + lists[1].size();
+  int4 *buf = pl->wg_buf;
+  if (pl->static_lists) {
+    // the library's static array: no allocation, a size limit
+    if (n > kStaticListEntries)
+      return set_error(FDTD2D_EINVAL, "the tile lists of this launch shape take " + std::to_string(n * sizeof(int4)) +
+                                          " bytes, beyond the " + std::to_string(kStaticListEntries * sizeof(int4)) +
+                                          "-byte static list array of fdtd2d_plan_create_in");
+  } else {
+    buf = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&buf), std::max<size_t>(n, 1) * sizeof(int4)),
+            "hipMalloc tile lists");
+  }
   // the launches queued so far read the current lists
   if (pl->wg_buf) {
-    HIP_TRY(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
-    hipFree(pl->wg_buf);
-    pl->wg_buf = nullptr;
+    hipError_t e = hipStreamSynchronize(pl->stream);
+    if (e != hipSuccess) {
+      if (!pl->static_lists) hipFree(buf);
+      return hip_fail(e, "hipStreamSynchronize");
+    }
   }
-  HIP_TRY(hipMalloc(reinterpret_cast<void **>(&pl->wg_buf), std::max<size_t>(n, 1) * sizeof(int4)),
-          "hipMalloc tile lists");
-  HIP_TRY(hipMemcpy(pl->wg_buf, lists[0].data(), lists[0].size() * sizeof(int4), hipMemcpyHostToDevice),
-          "hipMemcpy lists");
-  HIP_TRY(hipMemcpy(pl->wg_buf + lists[0].size(), lists[1].data(), lists[1].size() * sizeof(int4),
-                    hipMemcpyHostToDevice),
-          "hipMemcpy lists");
-  a.tile_list[0] = pl->wg_buf;
-  a.tile_list[1] = pl->wg_buf + lists[0].size();
+  hipError_t e = hipMemcpy(buf, lists[0].data(), lists[0].size() * sizeof(int4), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(buf + lists[0].size(), lists[1].data(), lists[1].size() * sizeof(int4), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (!pl->static_lists) hipFree(buf);
+    return hip_fail(e, "hipMemcpy tile lists");
+  }
+  if (!pl->static_lists && pl->wg_buf) hipFree(pl->wg_buf);
+  pl->wg_buf = buf;
+  commit(a);
+  a.tile_list[0] = buf;
+  a.tile_list[1] = buf + lists[0].size();
   a.n_wg[0] = static_cast<int>(lists[0].size()) / fdtd2d::kWavesPerWG;
   a.n_wg[1] = static_cast<int>(lists[1].size()) / fdtd2d::kWavesPerWG;
   return FDTD2D_OK;
@@ -635,6 +696,37 @@ int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out) {
   return create_plan(p, device, false, nullptr, 0, C, 0, 0, nullptr, out);
 }
 
+int fdtd2d_plan_create_in(const fdtd2d_problem *p, int device, void *mem, size_t bytes, void *stream,
+                          fdtd2d_plan **out) {
+  if (!out) return set_error(FDTD2D_EINVAL, "null out");
+  *out = nullptr;
+  if (!p) return set_error(FDTD2D_EINVAL, "null problem");
+  if (!mem) return set_error(FDTD2D_EINVAL, "null device memory");
+  if (reinterpret_cast<uintptr_t>(mem) % kAlign)
+    return set_error(FDTD2D_EINVAL, "the plan's memory must be 256-byte aligned");
+  int rc = fdtd2d::validate_problem(p);
+  if (rc) return rc;
+  const size_t need = plan_bytes(p);
+  if (bytes < need)
+    return set_error(FDTD2D_EINVAL, "the plan needs " + std::to_string(need) + " bytes of device memory (" +
+                                        std::to_string(bytes) + " given; fdtd2d_plan_footprint)");
+  if (device < 0 || device >= kMaxDevices) return set_error(FDTD2D_EINVAL, "no such device");
+  {
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (g_static_busy[device])
+      return set_error(FDTD2D_ESTATE, "another plan made by fdtd2d_plan_create_in exists on this device (they "
+                                      "share its static tile-list array)");
+    g_static_busy[device] = true;
+  }
+  const int C = default_cols_per_lane(p->nx, p->ny);
+  rc = create_plan(p, device, false, static_cast<hipStream_t>(stream), 0, C, 0, 0, static_cast<char *>(mem), out,
+                   true);
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (rc) g_static_busy[device] = false;  // a plan holds the array only once it exists
+  else (*out)->holds_static = true;
+  return rc;
+}
+
 }  // extern "C"
 
 size_t fdtd2d::detail::plan_bytes(const fdtd2d_problem *p) {
@@ -642,7 +734,7 @@ size_t fdtd2d::detail::plan_bytes(const fdtd2d_problem *p) {
 }
 
 int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled, hipStream_t stream, int T, int C,
-                                int band, int band_cpml, char *base, fdtd2d_plan **out) {
+                                int band, int band_cpml, char *base, fdtd2d_plan **out, bool caller_mem) {
   if (!out) return set_error(FDTD2D_EINVAL, "null out");
   *out = nullptr;
   int rc = fdtd2d::validate_problem(p);
@@ -686,7 +778,13 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     fdtd2d_plan_destroy(pl);
     return code;
   };
-  if (pooled) {
+  if (caller_mem) {
+    // the caller's memory and stream; tile lists in the static array
+    pl->stream = stream;
+    pl->static_lists = true;
+    rc = static_lists_ptr(device, &pl->wg_buf);
+    if (rc) return fail(rc);
+  } else if (pooled) {
     // a pool orders its shots' work itself (events on its own streams); the
     // plan's copies go to the null stream, which never waits for the pool's
     (void)stream;
@@ -700,7 +798,15 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       return fail(set_error(FDTD2D_EDEVICE, "hipEventCreate"));
     pl->stream = pl->own_stream;
   }
-  if (hipMemset(pl->base, 0, L.total) != hipSuccess) return fail(set_error(FDTD2D_EDEVICE, "hipMemset"));
+  if (caller_mem) {
+    // L.total is a multiple of 256 bytes (make_layout)
+    hipLaunchKernelGGL(zero_words, dim3(1024), dim3(256), 0, pl->stream, reinterpret_cast<uint4 *>(pl->base),
+                       L.total / 16);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(pl->stream) != hipSuccess)
+      return fail(set_error(FDTD2D_EDEVICE, "zeroing the plan's memory"));
+  } else if (hipMemset(pl->base, 0, L.total) != hipSuccess) {
+    return fail(set_error(FDTD2D_EDEVICE, "hipMemset"));
+  }
 
   const int nx = L.nx, ny = L.ny, pitch = L.pitch, P = L.P;
   // medium in pitch layout
@@ -1013,14 +1119,33 @@ int fdtd2d_plan_set_stream(fdtd2d_plan *pl, void *stream) {
 int fdtd2d_plan_set_tblock(fdtd2d_plan *pl, int T) {
   if (!pl) return set_error(FDTD2D_EINVAL, "null plan");
   if (T < 0 || T > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "steps per launch must be 0..8");
+  const int old = pl->T;
   pl->T = T;
-  return set_tiling(pl);
+  const int rc = set_tiling(pl);
+  if (rc) pl->T = old;
+  return rc;
 }
 
 int fdtd2d_plan_set_option(fdtd2d_plan *pl, const char *name, double value) {
   if (!pl || !name) return set_error(FDTD2D_EINVAL, "null argument");
   std::string n(name);
   const int v = static_cast<int>(value);
+  // the shape before the change, restored if the new one cannot be set up
+  const int oT = pl->T, oband = pl->band, oband_cpml = pl->band_cpml, oC = pl->C, oD = pl->D,
+            ojz = pl->jz_first, ofused = pl->fused;
+  auto retile = [&]() {
+    const int rc = set_tiling(pl);
+    if (rc) {
+      pl->T = oT;
+      pl->band = oband;
+      pl->band_cpml = oband_cpml;
+      pl->C = oC;
+      pl->D = oD;
+      pl->jz_first = ojz;
+      pl->fused = ofused;
+    }
+    return rc;
+  };
   if (n == "tblock") {
     if (v < 0 || v > fdtd2d::kMaxT) return set_error(FDTD2D_EINVAL, "tblock out of range");
     pl->T = v;
@@ -1049,7 +1174,7 @@ int fdtd2d_plan_set_option(fdtd2d_plan *pl, const char *name, double value) {
   } else {
     return set_error(FDTD2D_EINVAL, "unknown option " + n);
   }
-  return set_tiling(pl);
+  return retile();
 }
 
 int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value) {
@@ -1103,8 +1228,18 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       if (!e0 || !e1 || !c0 || !c1) return set_error(FDTD2D_EDEVICE, "hipEventCreate");
     }
     // CPML work-groups on the aux stream, interior ones on the main stream;
-    // both read buffer set src and write the other, so they may overlap.
-    if (pl->a.n_wg[1] > 0) {
+    // both read buffer set src and write the other, so they may overlap. A
+    // plan with no aux stream (fdtd2d_plan_create_in) runs the two kernels
+    // one after the other on its stream.
+    if (pl->a.n_wg[1] > 0 && !pl->aux) {
+      if (pl->timing) HIP_TRY(hipEventRecord(c0, pl->stream), "hipEventRecord");
+      hipError_t e = fdtd2d::launch_step(pl->a, 1, t, pl->C, 1, pl->src, pl->n, pl->stream);
+      if (e != hipSuccess) return hip_fail(e, "CPML step kernel launch");
+      if (pl->timing) {
+        HIP_TRY(hipEventRecord(c1, pl->stream), "hipEventRecord");
+        pl->pending[1].push_back({c0, c1});
+      }
+    } else if (pl->a.n_wg[1] > 0) {
       HIP_TRY(hipEventRecord(pl->ev_fork, pl->stream), "hipEventRecord fork");
       HIP_TRY(hipStreamWaitEvent(pl->aux, pl->ev_fork, 0), "hipStreamWaitEvent fork");
       if (pl->timing) HIP_TRY(hipEventRecord(c0, pl->aux), "hipEventRecord");
@@ -1128,7 +1263,7 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       HIP_TRY(hipEventRecord(e1, pl->stream), "hipEventRecord");
       pl->pending[0].push_back({e0, e1});
     }
-    if (pl->a.n_wg[1] > 0) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
+    if (pl->a.n_wg[1] > 0 && pl->aux) HIP_TRY(hipStreamWaitEvent(pl->stream, pl->ev_join, 0), "hipStreamWaitEvent join");
     if (pl->nrc > 0) {
       e = fdtd2d::launch_rc(pl->rc, pl->n, t, pl->stream);
       if (e != hipSuccess) return hip_fail(e, "RC kernel launch");
@@ -1333,7 +1468,7 @@ int fdtd2d_plan_kernel_ms(fdtd2d_plan *pl, double *avg_ms, int *count) {
 int fdtd2d_plan_destroy(fdtd2d_plan *pl) {
   if (!pl) return FDTD2D_OK;
   hipSetDevice(pl->device);
-  if (pl->stream) hipStreamSynchronize(pl->stream);
+  if (pl->stream || pl->static_lists) hipStreamSynchronize(pl->stream);
   for (int c = 0; c < 2; ++c)
     for (auto &pr : pl->pending[c]) {
       hipEventDestroy(pr.first);
@@ -1343,7 +1478,12 @@ int fdtd2d_plan_destroy(fdtd2d_plan *pl) {
   if (pl->aux) hipStreamSynchronize(pl->aux);
   if (pl->ev_fork) hipEventDestroy(pl->ev_fork);
   if (pl->ev_join) hipEventDestroy(pl->ev_join);
-  if (pl->wg_buf) hipFree(pl->wg_buf);
+  if (pl->holds_static) {
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    g_static_busy[pl->device] = false;
+  } else if (pl->wg_buf && !pl->static_lists) {
+    hipFree(pl->wg_buf);
+  }
   if (pl->base && pl->owns_base) hipFree(pl->base);
   if (pl->aux) hipStreamDestroy(pl->aux);
   if (pl->own_stream) hipStreamDestroy(pl->own_stream);

@@ -97,7 +97,11 @@ int hip_fail(hipError_t e, const char *what);
 
 struct fdtd2d_plan {
   bool pooled = false;  // a pool shot: no streams of its own, host tile lists only
-  bool owns_base = true;  // false: base is carved from a pool's arena
+  bool owns_base = true;  // false: base is carved from a pool's arena or given by the caller
+  // created by fdtd2d_plan_create_in: tile lists in the library's static
+  // per-device array, no streams or events of its own
+  bool static_lists = false;
+  bool holds_static = false;  // the plan owns its device's static array (released by destroy)
   fdtd2d::detail::Tiling tiling;  // pooled: the tile lists the pool concatenates
   int device = 0;
   hipStream_t stream = nullptr;
@@ -162,8 +166,11 @@ namespace detail {
 // takes; work goes to `stream`.
 // base: device memory of at least make_layout(p, events).total bytes to use
 // instead of a hipMalloc of its own (nullptr: allocate).
+// caller_mem: base is the caller's device memory and stream the caller's
+// stream; the plan creates no stream or event and allocates no device
+// memory (its tile lists go to the static array, fdtd2d_plan_create_in).
 int create_plan(const fdtd2d_problem *p, int device, bool pooled, hipStream_t stream, int T, int C, int band,
-                int band_cpml, char *base, fdtd2d_plan **out);
+                int band_cpml, char *base, fdtd2d_plan **out, bool caller_mem = false);
 // Bytes create_plan needs for p (the plan's own allocation).
 size_t plan_bytes(const fdtd2d_problem *p);
 int set_tiling(fdtd2d_plan *pl);

@@ -359,6 +359,29 @@ class Plan:
         if tblock:
             self.set_tblock(tblock)
 
+    @classmethod
+    def in_memory(cls, problem, device: int, mem_ptr: int, nbytes: int, stream_ptr: int = 0) -> "Plan":
+        """A plan in device memory the caller owns (fdtd2d_plan_create_in):
+        no device allocation, work on the given stream (0: the null stream);
+        at most one such plan per device at a time."""
+        self = cls.__new__(cls)
+        if isinstance(problem, Setup):
+            self._setup, self._keep = problem, None
+            ptr = problem.c_problem
+            s = ptr.contents
+        else:
+            s, self._keep = problem.to_c()
+            ptr = C.byref(s)
+            self._setup = None
+        self.nx, self.ny, self.npml, self.nsteps = s.nx, s.ny, s.npml, s.nsteps
+        self.ndet, self.ncomp, self.nrc = s.ndet, s.ncomp, s.nrc
+        self.line_shape = line_shape(s)
+        h = C.c_void_p()
+        L.check(L.lib().fdtd2d_plan_create_in(ptr, device, C.c_void_p(mem_ptr), nbytes, C.c_void_p(stream_ptr),
+                                              C.byref(h)), "fdtd2d_plan_create_in")
+        self._h = h
+        return self
+
     def set_tblock(self, t: int) -> None:
         L.check(L.lib().fdtd2d_plan_set_tblock(self._h, t), "fdtd2d_plan_set_tblock")
 

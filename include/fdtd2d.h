@@ -280,6 +280,25 @@ int fdtd2d_snapshots_close(fdtd2d_snapshots *w, int *saved);
 
 /* Copies the problem to device `device`; all state starts at zero. */
 int fdtd2d_plan_create(const fdtd2d_problem *p, int device, fdtd2d_plan **out);
+/* As fdtd2d_plan_create, allocating nothing on the device and creating no
+ * stream, event or graph. No counterpart in the reference: for callers that
+ * cannot call hipMalloc (code under stream capture, serving loops that carve
+ * shots from an arena of their own, harnesses that forbid allocation).
+ *   mem, bytes: device memory on `device`, 256-byte aligned, at least
+ *     fdtd2d_plan_footprint(p) bytes, owned by the caller and used by the
+ *     plan until fdtd2d_plan_destroy (zeroed here, on `stream`, by a kernel);
+ *   stream: the hipStream_t all of the plan's work goes to (NULL: the null
+ *     stream); the plan's second kernel, where the grid needs one, runs after
+ *     the first on that stream instead of beside it on a stream of its own.
+ * The step kernels' tile lists live in a static 1 MiB device array of the
+ * library (65536 entries of 16 bytes), one per device, held until the
+ * process ends: a launch shape needing more is FDTD2D_EINVAL (here or at
+ * fdtd2d_plan_set_option), and since every such plan on a device would share
+ * that array, only one may exist per device at a time (FDTD2D_ESTATE);
+ * plans of fdtd2d_plan_create are unaffected. Everything else (run, read,
+ * write, options, destroy, which frees nothing) works as for any plan. */
+int fdtd2d_plan_create_in(const fdtd2d_problem *p, int device, void *mem, size_t bytes, void *stream,
+                          fdtd2d_plan **out);
 /* Work on `stream` (a hipStream_t; NULL = the plan's own stream). */
 int fdtd2d_plan_set_stream(fdtd2d_plan *plan, void *stream);
 /* Time-blocking factor per launch (1..8; 0 = autotuned default). */
