@@ -155,6 +155,7 @@ SIGNATURES = [
     ("fdtd2d_setup_get", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     ("fdtd2d_setup_destroy", None, [C.c_void_p]),
     ("fdtd2d_write_outputs", C.c_int, [C.c_void_p, C.c_char_p, _f, _f, C.c_double]),
+    ("fdtd2d_write_ini_log", C.c_int, [C.c_void_p, C.c_char_p]),
     ("fdtd2d_write_rc_outputs", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32)]),
     ("fdtd2d_read_rc", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_size_t]),
     ("fdtd2d_read_line", C.c_int, [C.c_void_p, _f, C.c_size_t]),

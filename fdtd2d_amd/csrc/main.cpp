@@ -79,6 +79,7 @@ int main(int argc, char **argv) {
   int rc = fdtd2d_setup_from_ini(ini.c_str(), &s);
   if (rc) return die(rc, "setup");
   if (fdtd2d_setup_ini_status(s) < 0) std::printf("Can't load '%s'\n", ini.c_str());
+  if ((rc = fdtd2d_write_ini_log(s, out.c_str()))) return die(rc, "write_ini_log");
   const fdtd2d_problem *p = fdtd2d_setup_problem(s);
   double gpu = 0, s2d = 0;
   fdtd2d_setup_get(s, "GPU_device", &gpu);

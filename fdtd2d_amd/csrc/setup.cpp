@@ -6,8 +6,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <new>
 #include <random>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -196,7 +198,7 @@ void read_settings(const IniFile &r, RefSettings &s) {
   s.dt_coef = r.get_real("grid", "dt_coef", 0.5);
   s.Imax = r.get_int("grid", "Imax", 550);
   s.Jmax = r.get_int("grid", "Jmax", 600);
-  s.Exp_time_key = r.get_int("grid", "Use_Exp_time", 0);
+  s.Exp_time_key = r.get_int("grid", "Use_Exp_time", 0) != 0;  // a bool in the reference
   s.nMax = r.get_int("grid", "Max_time", 10000);
   s.Exp_time = r.get_real("grid", "Exp_time", 10);
   s.srcI = r.get_int("grid", "Isource", 300);
@@ -290,6 +292,67 @@ void aesa_array(int PI, int PK, int DI, int DK, int cenI, int cenK, std::vector<
   for (int k = 0; k < qK; ++k) arrK[k] = Kstart + dk * k;
 }
 
+// The configuration echo logINIread appends to iniLOG.txt after reading the
+// INI, before any unit conversion (F/FDTD_cuda.cu:178-303): the same keys in
+// the same order, each value printed by an ostream as the reference's
+// variable's type prints it (bool as 0/1, float and double in the default
+// 6-significant-digit format), and the time and date of the read as its
+// _strtime_s / _strdate_s give them (HH:MM:SS, MM/DD/YY).
+std::string ini_log_text(const RefSettings &s) {
+  char tm_s[32], dt_s[32];
+  const std::time_t now = std::time(nullptr);
+  std::strftime(tm_s, sizeof tm_s, "%H:%M:%S", std::localtime(&now));
+  std::strftime(dt_s, sizeof dt_s, "%m/%d/%y", std::localtime(&now));
+  const MediumParams &m = s.med;
+  const fdtd2d_processing &a = s.proc;
+  const bool exp_time = s.Exp_time_key != 0;
+  std::ostringstream o;
+  o << "Config loaded from 'Settings.ini' on time " << tm_s << " date " << dt_s << ":\nHardware:"
+    << "\n GPU_device=" << s.GPU_device << "\nRegimes: "
+    << "\n SourceToDetect=" << s.S2D << "\nGrid param: "
+    << "\n Use_Exp_time=" << exp_time << "\n Imax=" << s.Imax << "\n Jmax=" << s.Jmax << "\n Max_time=" << s.nMax
+    << "\n Exp_time=" << s.Exp_time << "\n ds=" << s.ds << "\n dt_coef=" << s.dt_coef << "\n Isource=" << s.srcI
+    << "\n Jsource=" << s.srcJ << "\n Idetect=" << s.detI << "\n Jdetect=" << s.detJ
+    << "\nAESA\n SIsize=" << s.sPI << "\n SAutoDelta=" << s.sAuto << "\n SIdelta=" << s.sDI
+    << "\n SPAngle=" << s.angle << "\n DIsize=" << s.dPI << "\n DAutoDelta=" << s.dAuto << "\n DIdelta=" << s.dDI
+    << "\nPulseParam:"
+    << "\n Chirp=" << s.chirp << "\n FreqA=" << s.freqA << "\n FreqB=" << s.freqB << "\n Fronts=" << s.front
+    << "\n Plato=" << s.plato << "\n Fronts_ns=" << s.fronts_ns << "\n Plato_ns=" << s.plato_ns
+    << "\n Pulse_amp=" << s.amp << "\n2Dsave:"
+    << "\n From=" << s.fromT << "\n To=" << s.toT << "\n TimeSieve=" << s.timeSieve
+    << "\n GridSieve=" << s.fieldSieve << "\nmedium:"
+    << "\n Build=" << m.build << "\n Icen=" << m.cenI << "\n Jcen=" << m.cenJ
+    << "\n\n A=" << m.A << "\n B=" << m.B << "\n C=" << m.C << "\n P=" << m.P << "\n Q=" << m.Q << "\n N=" << m.N
+    << "\n\n Radius=" << m.R << "\n Alpha=" << m.alpha << "\n Beta=" << m.beta << "\n ne0=" << m.ne0
+    << "\n neA=" << m.neA << "\nmediumFluc:"
+    << "\n FlucBuild=" << m.fluc_build << "\n FlucIcen=" << m.fcenI << "\n FlucJcen=" << m.fcenJ
+    << "\n\n FlucA=" << m.fA << "\n FlucB=" << m.fB << "\n FlucC=" << m.fC << "\n FlucP=" << m.fP
+    << "\n FlucQ=" << m.fQ << "\n FlucN=" << m.fN
+    << "\n\n FlucAngleI=" << m.angI << "\n FlucAngleJ=" << m.angJ << "\n FlucAngleK=" << m.angK
+    << "\n FlucSign=" << m.sign
+    << "\n\n FlucRadius=" << m.fR << "\n FlucAlpha=" << m.falpha << "\n FlucBeta=" << m.fbeta
+    << "\n FlucNe0=" << m.fne0
+    << "\n TorusBR=" << m.torusR << "\nKameraConf:"
+    << "\n Draw2D=" << s.draw2d << "\n 2Dpreset=" << s.preset << "\n PatrubokL=" << s.patL
+    << "\n DrawWall=" << s.drawWall << "\n WallPos=" << s.wallPos
+    << "\n\nRC_calc:"
+    << "\n Iline=" << s.rcI << "\n LIsize=" << s.rcPI << "\n LIdelta=" << s.rcDI << "\n FilterWindow=" << s.rcWindow
+    << "\n Rlevel=" << s.rcLevel
+    << "\n\nProcessing:"
+    << "\n Perform=" << a.perform << "\n SecDer=" << a.sec_der << "\n SDFType=" << a.sdf_type
+    << "\n SDFWin=" << a.sdf_win << "\n SDFRuns=" << a.sdf_runs << "\n CF=" << a.cf << "\n CFTau=" << a.cf_tau
+    << "\n CFK=" << a.cf_k << "\n ARC=" << a.arc << "\n ARCTau=" << a.arc_tau << "\n ARCK=" << a.arc_k
+    << "\n TauThreshold=" << a.tau_threshold << "\n FilterType=" << a.filter_type << "\n FilterWin=" << a.filter_win
+    << "\n FilterRuns=" << a.filter_runs << "\n SaveFiltered=" << a.save_filtered
+    << "\n SaveOriginal=" << a.save_original << "\n PosI=" << a.pos_i << "\n PosJ=" << a.pos_j
+    << "\n PosK=" << a.pos_k << "\n SizeI=" << a.size_i << "\n BeginAfter=" << a.begin_after
+    << "\n FinishAt=" << a.finish_at << "\n TimeSieve=" << a.time_sieve << "\n DimSieve=" << a.dim_sieve
+    << "\n SimDetect=" << a.sim_detect << "\n DetSave=" << a.save_detector << "\n DetSize=" << a.detector_size
+    << "\n DetWLoad=" << a.load_weights
+    << "\n-----------------------------------------------------------------------------\n";
+  return o.str();
+}
+
 }  // namespace
 
 int setup_from_ini(const std::string &path, Setup &S, bool objects) {
@@ -298,7 +361,10 @@ int setup_from_ini(const std::string &path, Setup &S, bool objects) {
   RefSettings st;
   // A file that cannot be opened leaves every compile-time default in place
   // (F/FDTD_cuda.cu:30-33 returns before any Get call); parse errors do not.
-  if (r.status() >= 0) read_settings(r, st);
+  if (r.status() >= 0) {
+    read_settings(r, st);
+    S.ini_log = ini_log_text(st);  // the reference logs a read that opened the file
+  }
   // recuclParametersToMetric, F/FDTD_cuda.cu:305-382: int fields are scaled by
   // a double 1/ds and truncated on assignment.
   const double ds = st.ds;

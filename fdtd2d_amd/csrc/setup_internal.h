@@ -45,6 +45,7 @@ struct Setup {
   std::vector<float> src_table;
   std::vector<float> source_profile;  // SourceP[nMax] envelope (SourceP.txt)
   std::map<std::string, double> params;
+  std::string ini_log;  // iniLOG.txt text of the INI read (empty for other setups)
   std::string description;
   int ini_status = 0;
   int det_rows = 0, det_cols = 1;  // detector grid (det_qantI, det_qantK) for writers

@@ -154,6 +154,50 @@ extern "C" int fdtd2d_write_outputs(const fdtd2d_setup *setup, const char *dir, 
     std::fprintf(f, "\n\nCalculation on GPU\n Calculation time: %f seconds\n\n", seconds);
     std::fclose(f);
   }
+  // deb.txt, the material id (ID3) map DevFieldCalculation prints after the
+  // run (F/kernel.cu:2683, 2737-2751): one line per column i in 1..Imax-2 of
+  // "%d " for rows j in 1..Jmax-2, then two blank-line pairs; a problem with
+  // no id map (density medium) has none to print. materials.txt is opened
+  // beside it and left empty (its writes are commented out, :2685-2731).
+  if (!S.material.empty()) {
+    FILE *f = open_in(d, "deb.txt", "w");
+    if (!f) return set_error(FDTD2D_EIO, "cannot write deb.txt in " + d);
+    const int Imax = p.nx, Jmax = p.ny;
+    std::string line;
+    char num[16];
+    for (int i = 1; i < Imax - 1; ++i) {
+      line.clear();
+      for (int j = 1; j < Jmax - 1; ++j) {
+        std::snprintf(num, sizeof num, "%d ", S.material[static_cast<size_t>(j) * Imax + i]);
+        line += num;
+      }
+      line += '\n';
+      std::fwrite(line.data(), 1, line.size(), f);
+    }
+    std::fputs("\n\n\n\n", f);
+    std::fclose(f);
+  }
+  // files DevFieldCalculation opens and never writes in this configuration
+  // (F/kernel.cu:1948-1993, 2685): the debug and line-detector files, the
+  // Ex/Ey/Hz component files, the difference file (written afterwards by the
+  // source-to-detector regime only) and the composite-detector file
+  for (const char *name : {"test.txt", "SDetect.txt", "LineDetect.txt", "LineDetect2.txt", "detEx.txt", "detEy.txt",
+                           "detHz.txt", "DifDetEx.txt", "CompDetect.txt", "materials.txt"})
+    if (int rc = touch(d, name)) return rc;
+  return FDTD2D_OK;
+}
+
+// iniLOG.txt (append): the configuration echo of the reference's INI read
+// (logINIread, F/FDTD_cuda.cu:178-303), captured when the setup read it.
+extern "C" int fdtd2d_write_ini_log(const fdtd2d_setup *setup, const char *dir) {
+  using fdtd2d::set_error;
+  if (!setup || !dir) return set_error(FDTD2D_EINVAL, "null argument");
+  const std::string &text = setup->s.ini_log;
+  if (text.empty()) return FDTD2D_OK;  // the INI could not be opened (the reference logs nothing then)
+  FILE *f = open_in(std::string(dir), "iniLOG.txt", "a");
+  if (!f) return set_error(FDTD2D_EIO, std::string("cannot append iniLOG.txt in ") + dir);
+  std::fwrite(text.data(), 1, text.size(), f);
+  std::fclose(f);
   return FDTD2D_OK;
 }
 

@@ -280,6 +280,10 @@ class Setup:
         L.check(L.lib().fdtd2d_write_rc_outputs(self._h, str(out_dir).encode(), v.ctypes.data_as(C.POINTER(C.c_int32))),
                 "fdtd2d_write_rc_outputs")
 
+    def write_ini_log(self, out_dir: str) -> None:
+        """Appends the INI read's configuration echo to out_dir/iniLOG.txt."""
+        L.check(L.lib().fdtd2d_write_ini_log(self._h, str(out_dir).encode()), "fdtd2d_write_ini_log")
+
     def write_outputs(self, out_dir: str, det: np.ndarray, comp: np.ndarray, seconds: float = 0.0) -> None:
         d = np.ascontiguousarray(det, np.float32)
         c = np.ascontiguousarray(comp, np.float32)

@@ -249,6 +249,16 @@ int fdtd2d_processing_tof(const fdtd2d_processing *a, const float *line, int tim
 int fdtd2d_write_processing_outputs(const fdtd2d_processing *a, const char *dir, const float *line, int times);
 int fdtd2d_write_outputs(const fdtd2d_setup *s, const char *dir, const float *det,
                          const float *comp, double seconds);
+/* iniLOG.txt (appended): the configuration echo the reference writes after
+ * reading Settings.ini (logINIread, F/FDTD_cuda.cu:178-303), with the keys'
+ * values as read (before the conversion to cells) and the time of the read;
+ * nothing when the setup did not come from an INI file that could be opened.
+ * (fdtd2d_write_outputs also writes deb.txt, the material id map
+ * F/kernel.cu:2737-2751, for table-medium problems, and creates the empty
+ * files the reference opens: test.txt, SDetect.txt, LineDetect.txt,
+ * LineDetect2.txt, detEx/Ey/Hz.txt, DifDetEx.txt, CompDetect.txt,
+ * materials.txt.) */
+int fdtd2d_write_ini_log(const fdtd2d_setup *s, const char *dir);
 /* Source-to-detector regime ([regimes] SourceToDetect = 1, F/FDTD_cuda.cu:
  * 389-419): the free-space pass's files (Detect_S2D.txt, detAll_S2D.txt,
  * detEz/Hx/Hy_S2D.txt and the empty files the reference opens,

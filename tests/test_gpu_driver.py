@@ -4,7 +4,8 @@ Each shipped INI (medium builders: sphere, torus, fluctuation) is set up
 exactly as the fdtd2d CLI does (fdtd2d_setup_from_ini), stepped on the GPU for
 a bounded number of steps and compared with the FP32 oracle: traces bitwise,
 and the output files the reference's scripts read (Detect.txt, detAll.txt,
-detEz/Hx/Hy.txt, detP.txt, SourceP.txt) byte for byte.
+detEz/Hx/Hy.txt, detP.txt, SourceP.txt) byte for byte; the CLI runs also
+write deb.txt, iniLOG.txt and the empty files the reference opens.
 """
 
 from __future__ import annotations
@@ -22,6 +23,9 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).parent / "golden"
 STEPS = 600
 FILES = ["Detect.txt", "detAll.txt", "detEz.txt", "detHx.txt", "detHy.txt", "detP.txt", "SourceP.txt"]
+# deb.txt (the material id map) and the files the reference opens and leaves empty
+DEBUG_FILES = ["deb.txt", "materials.txt", "test.txt", "SDetect.txt", "LineDetect.txt", "LineDetect2.txt",
+               "detEx.txt", "detEy.txt", "detHz.txt", "CompDetect.txt"]
 
 
 @pytest.mark.parametrize("name", ["settings_full.ini", "settings_release.ini", "settings_build.ini"])
@@ -219,9 +223,15 @@ def test_cli_snapshots_and_chamber_figure_match_oracle(gpu_device, tmp_path):
     assert w.close() == 4
     s.set("SlicesSaved", 4)
     s.write_outputs(str(r), det, comp)
-    for f in ["Ez.txt", "Ex.txt", "Ey.txt", "Hx.txt", "Hy.txt", "Hz.txt", "FieldSaveDat.txt"] + FILES:
+    for f in ["Ez.txt", "Ex.txt", "Ey.txt", "Hx.txt", "Hy.txt", "Hz.txt", "FieldSaveDat.txt"] + FILES + DEBUG_FILES:
         assert (g / f).read_bytes() == (r / f).read_bytes(), f
     assert "Slises saved: 4" in (g / "log.txt").read_text()
+    assert (g / "deb.txt").stat().st_size > 0 and "5 " in (g / "deb.txt").read_text()
+    # iniLOG.txt: the INI read's echo, once per run (time of the read aside)
+    s.write_ini_log(str(r))
+    strip = lambda t: t.split("\n", 1)[1]  # noqa: E731
+    gl, rl = (g / "iniLOG.txt").read_text(), (r / "iniLOG.txt").read_text()
+    assert gl.count("Config loaded from 'Settings.ini'") == 1 and strip(gl) == strip(rl)
 
 
 def test_cli_source_to_detector_regime_matches_oracle(gpu_device, tmp_path):
@@ -245,7 +255,7 @@ def test_cli_source_to_detector_regime_matches_oracle(gpu_device, tmp_path):
     s.write_outputs(str(r), det, comp)
     s.write_s2d_difference(str(r), comp, comp0)
     names = ["Detect_S2D.txt", "detAll_S2D.txt", "detEz_S2D.txt", "detHx_S2D.txt", "detHy_S2D.txt", "detP_S2D.txt",
-             "CompDetect_S2D.txt", "DifDetEx.txt"] + FILES
+             "CompDetect_S2D.txt", "DifDetEx.txt"] + FILES + DEBUG_FILES
     for f in names:
         assert (g / f).read_bytes() == (r / f).read_bytes(), f
     dif = np.loadtxt(g / "DifDetEx.txt")
