@@ -532,12 +532,10 @@ int set_tiling(fdtd2d_plan *pl) {
     // ones (event entries only with no captures); the rest keep the
     // CPML-class kernel
     const bool captures = pl->a.nprobe > 0 || pl->line_rows > 0;
-    const int kinds = fdtd2d::fused_kinds(pl->a.medium_kind);
     std::vector<int4> xy;
     for (const int4 &e : t.lists[1]) {
       if (e.y >= e.z) continue;  // padding
-      const int k = fdtd2d::entry_kind(e.w);
-      const bool in = ((kinds >> k) & 1) && !(k == fdtd2d::kKindE && captures);
+      const bool in = fdtd2d::fusable_kind(fdtd2d::entry_kind(e.w), pl->a.medium_kind, captures);
       (in ? xy : lists[1]).push_back(e);
     }
     for (std::vector<int4> *v : {&xy, &lists[1]})
@@ -1233,7 +1231,7 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
     // one after the other on its stream.
     if (pl->a.n_wg[1] > 0 && !pl->aux) {
       if (pl->timing) HIP_TRY(hipEventRecord(c0, pl->stream), "hipEventRecord");
-      hipError_t e = fdtd2d::launch_step(pl->a, 1, t, pl->C, 1, pl->src, pl->n, pl->stream);
+      hipError_t e = fdtd2d::launch_step(pl->a, fused_launch(pl) ? 2 : 1, t, pl->C, 1, pl->src, pl->n, pl->stream);
       if (e != hipSuccess) return hip_fail(e, "CPML step kernel launch");
       if (pl->timing) {
         HIP_TRY(hipEventRecord(c1, pl->stream), "hipEventRecord");
@@ -1243,7 +1241,7 @@ int fdtd2d_run(fdtd2d_plan *pl, int nsteps) {
       HIP_TRY(hipEventRecord(pl->ev_fork, pl->stream), "hipEventRecord fork");
       HIP_TRY(hipStreamWaitEvent(pl->aux, pl->ev_fork, 0), "hipStreamWaitEvent fork");
       if (pl->timing) HIP_TRY(hipEventRecord(c0, pl->aux), "hipEventRecord");
-      hipError_t e = fdtd2d::launch_step(pl->a, 1, t, pl->C, 1, pl->src, pl->n, pl->aux);
+      hipError_t e = fdtd2d::launch_step(pl->a, fused_launch(pl) ? 2 : 1, t, pl->C, 1, pl->src, pl->n, pl->aux);
       if (e != hipSuccess) return hip_fail(e, "CPML step kernel launch");
       if (pl->timing) {
         HIP_TRY(hipEventRecord(c1, pl->aux), "hipEventRecord");

@@ -108,12 +108,15 @@ struct StepArgs {
 // items, count and g are written by kernels on the pool's stream, so every
 // launch sees the lists in stream order, also inside a replayed graph.
 constexpr int kMaxChunk = 16;
+// list classes: 0 interior, 1 CPML class (all of it, or in a fused pool the
+// fusable kinds), 2 the leftover CPML-class entries of a fused pool
+constexpr int kPoolLists = 3;
 struct PoolQueue {
-  const int4 *items[2];
-  int count[2];
+  const int4 *items[kPoolLists];
+  int count[kPoolLists];
   int g;
   int pad;
-  int ticket[kMaxChunk][2][8];
+  int ticket[kMaxChunk][kPoolLists][8];
 };
 
 constexpr int kRowRec = 12;  // ints per row record
@@ -137,19 +140,28 @@ constexpr int kKindShift = 2;
 inline __host__ __device__ int entry_kind(int flags) { return (flags >> kKindShift) & 7; }
 // A CPML-class entry in the one list of a fused single-grid launch.
 constexpr int kFlagCpml = 32;
+constexpr int kAllKinds = 31;
 // CPML-class kinds the fused single-grid kernel carries (bit 1 << kind) for
 // a medium (0 table, 1 density): those that keep it within 3 waves per SIMD
-// with no scratch spills at 8 steps per launch (with the corner kind the
-// table medium's kernel spills 46 VGPRs). Event entries of problems with RC
-// probes or a processing line never go there (their capturing row loop
-// would spill 180); the rest run in the CPML-class kernel beside it.
+// with no scratch spills at 8 steps per launch. The density medium's kernel
+// carries every kind (165 VGPRs); the table medium's only the strips and
+// the event kind without captures (with the corner kind it spills, and the
+// capturing row loop of event entries of problems with RC probes or a
+// processing line would too); its other entries run in a kernel beside it.
 constexpr int fused_kinds(int medium_kind) {
-  return medium_kind == 1 ? (1 << kKindX) | (1 << kKindY) | (1 << kKindXY) | (1 << kKindE)
-                          : (1 << kKindX) | (1 << kKindY) | (1 << kKindE);
+  return medium_kind == 1 ? kAllKinds : (1 << kKindX) | (1 << kKindY) | (1 << kKindE);
+}
+// Whether a CPML-class entry of this kind goes to the fused kernel (else to
+// the leftover kernel beside it), for a problem with or without captures
+// (RC probes, a processing line).
+constexpr bool fusable_kind(int kind, int medium_kind, bool captures) {
+  return fused_kinds(medium_kind) == kAllKinds ||
+         (((fused_kinds(medium_kind) >> kind) & 1) && !(kind == kKindE && captures));
 }
 
-// Launches the step kernel of work-group class `cls` (0 interior, 1 CPML)
-// for T steps on `stream`, D rows of prefetch; src = buffer set read.
+// Launches the step kernel of work-group class `cls` (0 interior, 1 CPML,
+// every kind; 2 CPML, the leftover kinds beside a fused launch) for T steps
+// on `stream`, D rows of prefetch; src = buffer set read.
 hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src, int n0, hipStream_t stream);
 // Launches the fused single-grid kernel over the combined list (tile_list[0],
 // n_wg[0] work-groups: CPML-class entries flagged kFlagCpml first in each
@@ -168,17 +180,19 @@ struct RcArgs {
 };
 hipError_t launch_rc(const RcArgs &r, int n0, int t, hipStream_t stream);
 
-// Pool kernels: launch k of a chunk, class cls, T steps, on `grid`
-// persistent work-groups that take work-groups of tile entries (entry.w =
-// flags | shot slot << 8, one slot per work-group) from q's lists until they
-// are exhausted. tp = the pool's steps per full launch. Only C = 1;
+// Pool kernels: launch k of a chunk, T steps, on `grid` persistent
+// work-groups that take work-groups of tile entries (entry.w = flags | shot
+// slot << 8, one slot per work-group) from q's lists until they are
+// exhausted. cls: 0 interior (list 0), 1 CPML class (list 1, every kind),
+// 2 fused (list 1 of fusable kinds, then list 0), 3 the leftover CPML-class
+// entries (list 2). tp = the pool's steps per full launch. Only C = 1;
 // D = 1 or 2 for the interior class.
 hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D, int medium_kind,
                        int grid, hipStream_t stream);
 // Resets the tickets of launches [0, nk) and adds dg to q->g (one wave).
 hipError_t launch_pool_tick(PoolQueue *q, int nk, int dg, hipStream_t stream);
 // Points q at new lists (g >= 0 also sets q->g), in stream order.
-hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,
+hipError_t launch_pool_set(PoolQueue *q, const int4 *const items[kPoolLists], const int count[kPoolLists], int g,
                            hipStream_t stream);
 
 // True when tile tx (columns from x0 + tx*vw, vw stored, halo recomputed

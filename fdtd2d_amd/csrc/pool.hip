@@ -145,14 +145,17 @@ struct fdtd2d_pool {
   size_t cap_slots = 0;
   // main lists, double-buffered: a rebuild writes the buffer the launches in
   // flight do not read, once the last launch that read it has finished
+  // (lists: 0 interior, 1 CPML class or its fusable kinds, 2 the leftover
+  // CPML-class entries of a fused pool)
   struct ListBuf {
-    int4 *d[2] = {nullptr, nullptr};
-    size_t cap[2] = {0, 0};
+    int4 *d[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
+    size_t cap[fdtd2d::kPoolLists] = {0, 0, 0};
     int last_g = -1;  // last pool launch that read it
   } bufs[2];
   int cur = 0;
-  int4 *d_tail[2] = {nullptr, nullptr};
-  size_t cap_tail[2] = {0, 0};
+  int main_count[fdtd2d::kPoolLists] = {0, 0, 0};  // work-groups per list of the current main lists
+  int4 *d_tail[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
+  size_t cap_tail[fdtd2d::kPoolLists] = {0, 0, 0};
   fdtd2d::PoolQueue *d_q = nullptr, *d_qt = nullptr;
   // host bookkeeping
   int g = 0;                 // pool launches issued
@@ -174,27 +177,57 @@ namespace {
 
 void retire(fdtd2d_pool *pp, int wait_g);
 
-int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
-  std::vector<int4> lists[2];
-  for (int s : slots) {
-    const Tiling &t = pp->shots[s].pl->tiling;
-    for (int c = 0; c < 2; ++c)
-      for (const int4 &e : t.lists[c]) lists[c].push_back(make_int4(e.x, e.y, e.z, (e.w & 0xff) | (s << 8)));
+// Appends shot s's tile entries to the pool lists, tagged with its slot, each
+// list padded to whole work-groups (one shot per work-group). A fused pool
+// sends the CPML-class entries of the kinds its fused kernel carries to list
+// 1 and the rest to list 2; otherwise every CPML-class entry goes to list 1.
+void append_shot(const fdtd2d_pool *pp, int s, std::vector<int4> *lists) {
+  const fdtd2d_plan *pl = pp->shots[s].pl;
+  const Tiling &t = pl->tiling;
+  const bool captures = pl->a.nprobe > 0 || pl->line_rows > 0;
+  for (int c = 0; c < 2; ++c) {
+    const size_t start[fdtd2d::kPoolLists] = {lists[0].size(), lists[1].size(), lists[2].size()};
+    for (const int4 &e : t.lists[c]) {
+      if (e.y >= e.z) continue;  // padding
+      int to = c;
+      if (c == 1 && pp->fused && !fdtd2d::fusable_kind(fdtd2d::entry_kind(e.w), pl->a.medium_kind, captures))
+        to = 2;
+      lists[to].push_back(make_int4(e.x, e.y, e.z, (e.w & 0xff) | (s << 8)));
+    }
+    for (int l = 0; l < fdtd2d::kPoolLists; ++l)
+      while ((lists[l].size() - start[l]) % fdtd2d::kWavesPerWG) lists[l].push_back(make_int4(0, 0, 0, s << 8));
   }
+}
+
+// Copies the lists to device buffers d[], growing them as needed.
+int upload_lists(const std::vector<int4> *lists, int4 **d, size_t *cap, int count[fdtd2d::kPoolLists]) {
+  bool moved = false;
+  for (int c = 0; c < fdtd2d::kPoolLists; ++c) {
+    int rc = grow(d[c], cap[c], std::max<size_t>(lists[c].size(), 4), moved);
+    if (rc) return rc;
+    if (!lists[c].empty())
+      HIP_TRY(hipMemcpy(d[c], lists[c].data(), lists[c].size() * sizeof(int4), hipMemcpyHostToDevice),
+              "hipMemcpy pool lists");
+    count[c] = static_cast<int>(lists[c].size() / fdtd2d::kWavesPerWG);
+  }
+  return FDTD2D_OK;
+}
+
+int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
+  std::vector<int4> lists[fdtd2d::kPoolLists];
+  for (int s : slots) append_shot(pp, s, lists);
   const int nb = 1 - pp->cur;
   fdtd2d_pool::ListBuf &b = pp->bufs[nb];
   retire(pp, b.last_g + 1);  // launches that read this buffer have finished
-  bool moved = false;
-  for (int c = 0; c < 2; ++c) {
-    int rc = grow(b.d[c], b.cap[c], std::max<size_t>(lists[c].size(), 4), moved);
-    if (rc) return rc;
-    if (!lists[c].empty())
-      HIP_TRY(hipMemcpy(b.d[c], lists[c].data(), lists[c].size() * sizeof(int4), hipMemcpyHostToDevice),
-              "hipMemcpy pool lists");
-  }
-  hipError_t e = fdtd2d::launch_pool_set(pp->d_q, b.d[0], b.d[1], static_cast<int>(lists[0].size() / 4),
-                                         static_cast<int>(lists[1].size() / 4), -1, pp->stream);
+  int count[fdtd2d::kPoolLists];
+  int rc = upload_lists(lists, b.d, b.cap, count);
+  if (rc) return rc;
+  hipError_t e = fdtd2d::launch_pool_set(pp->d_q, b.d, count, -1, pp->stream);
   if (e != hipSuccess) return hip_fail(e, "pool set launch");
+  // a captured graph holds the leftover kernel's grid (none when list 2 is empty)
+  const int gc = pp->ncu * pp->wg_per_cu_cpml;
+  if (std::min(gc, count[2]) != std::min(gc, pp->main_count[2])) pp->graph_valid = false;
+  for (int c = 0; c < fdtd2d::kPoolLists; ++c) pp->main_count[c] = count[c];
   pp->cur = nb;
   pp->main_set = slots;
   pp->lists_valid = true;
@@ -245,24 +278,27 @@ int record_done(fdtd2d_pool *pp) {
   return FDTD2D_OK;
 }
 
-// One pool launch of the main lists at chunk position k (both classes).
+// One pool launch of the main lists at chunk position k (every class).
+// Unfused: the CPML-class kernel on the aux stream beside the interior one.
+// Fused: one kernel takes the fusable CPML-class work-groups first, then the
+// interior ones; the leftover CPML-class list, when it has work, runs in its
+// own kernel on the aux stream beside it.
 int launch_main(fdtd2d_pool *pp, int k) {
   const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
-  if (pp->fused) {
-    // one kernel takes the CPML-class work-groups first, then the interior ones
-    hipError_t e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 2, pp->T, pp->D, pp->medium_kind, gi,
-                                       pp->stream);
-    if (e != hipSuccess) return hip_fail(e, "pool kernel launch");
-    return FDTD2D_OK;
+  const bool side = !pp->fused || pp->main_count[2] > 0;
+  hipError_t e = hipSuccess;
+  if (side) {
+    HIP_TRY(hipEventRecord(pp->ev_fork, pp->stream), "hipEventRecord fork");
+    HIP_TRY(hipStreamWaitEvent(pp->aux, pp->ev_fork, 0), "hipStreamWaitEvent fork");
+    e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, pp->fused ? 3 : 1, pp->T, 1, pp->medium_kind,
+                            pp->fused ? std::min(gc, pp->main_count[2]) : gc, pp->aux);
+    if (e != hipSuccess) return hip_fail(e, "pool CPML kernel launch");
+    HIP_TRY(hipEventRecord(pp->ev_join, pp->aux), "hipEventRecord join");
   }
-  HIP_TRY(hipEventRecord(pp->ev_fork, pp->stream), "hipEventRecord fork");
-  HIP_TRY(hipStreamWaitEvent(pp->aux, pp->ev_fork, 0), "hipStreamWaitEvent fork");
-  hipError_t e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 1, pp->T, 1, pp->medium_kind, gc, pp->aux);
-  if (e != hipSuccess) return hip_fail(e, "pool CPML kernel launch");
-  HIP_TRY(hipEventRecord(pp->ev_join, pp->aux), "hipEventRecord join");
-  e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, 0, pp->T, pp->D, pp->medium_kind, gi, pp->stream);
+  e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, pp->fused ? 2 : 0, pp->T, pp->D, pp->medium_kind, gi,
+                          pp->stream);
   if (e != hipSuccess) return hip_fail(e, "pool kernel launch");
-  HIP_TRY(hipStreamWaitEvent(pp->stream, pp->ev_join, 0), "hipStreamWaitEvent join");
+  if (side) HIP_TRY(hipStreamWaitEvent(pp->stream, pp->ev_join, 0), "hipStreamWaitEvent join");
   return FDTD2D_OK;
 }
 
@@ -299,26 +335,22 @@ int run_tails(fdtd2d_pool *pp, const std::vector<int> &tails) {
   for (int s : tails) by_t[pp->shots[s].pl->nsteps - pp->shots[s].pl->n].push_back(s);
   for (auto &kv : by_t) {
     const int t = kv.first;
-    std::vector<int4> lists[2];
-    for (int s : kv.second) {
-      const Tiling &tl = pp->shots[s].pl->tiling;
-      for (int c = 0; c < 2; ++c)
-        for (const int4 &e : tl.lists[c]) lists[c].push_back(make_int4(e.x, e.y, e.z, (e.w & 0xff) | (s << 8)));
-    }
+    std::vector<int4> lists[fdtd2d::kPoolLists];
+    for (int s : kv.second) append_shot(pp, s, lists);
     // the tail buffers are reused: wait until the previous tail launch is done
     HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
-    bool moved = false;
-    for (int c = 0; c < 2; ++c) {
-      int rc = grow(pp->d_tail[c], pp->cap_tail[c], std::max<size_t>(lists[c].size(), 4), moved);
-      if (rc) return rc;
-      if (!lists[c].empty())
-        HIP_TRY(hipMemcpy(pp->d_tail[c], lists[c].data(), lists[c].size() * sizeof(int4), hipMemcpyHostToDevice),
-                "hipMemcpy tail lists");
-    }
-    const int n0 = static_cast<int>(lists[0].size() / 4), n1 = static_cast<int>(lists[1].size() / 4);
+    int count[fdtd2d::kPoolLists];
+    int rc = upload_lists(lists, pp->d_tail, pp->cap_tail, count);
+    if (rc) return rc;
+    const int n0 = count[0], n1 = count[1], n2 = count[2];
     const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
-    hipError_t e = fdtd2d::launch_pool_set(pp->d_qt, pp->d_tail[0], pp->d_tail[1], n0, n1, pp->g, pp->stream);
+    hipError_t e = fdtd2d::launch_pool_set(pp->d_qt, pp->d_tail, count, pp->g, pp->stream);
     if (pp->fused) {
+      // in stream order: the leftover kernel's tiles are disjoint from the
+      // fused kernel's, both read set src and write set 1 - src
+      if (e == hipSuccess && n2 > 0)
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 3, t, 1, pp->medium_kind, std::min(gc, n2),
+                                pp->stream);
       if (e == hipSuccess && n0 + n1 > 0)
         e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 2, t, pp->D, pp->medium_kind, std::min(gi, n0 + n1),
                                 pp->stream);
@@ -737,7 +769,7 @@ int fdtd2d_pool_destroy(fdtd2d_pool *pp) {
   for (auto &de : pp->done_events) hipEventDestroy(de.second);
   for (auto e : pp->spare_events) hipEventDestroy(e);
   if (pp->gexec) hipGraphExecDestroy(pp->gexec);
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < fdtd2d::kPoolLists; ++c) {
     for (int b = 0; b < 2; ++b)
       if (pp->bufs[b].d[c]) hipFree(pp->bufs[b].d[c]);
     if (pp->d_tail[c]) hipFree(pp->d_tail[c]);

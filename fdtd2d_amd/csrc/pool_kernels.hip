@@ -10,19 +10,21 @@ namespace {
 
 __global__ void __launch_bounds__(64) pool_tick(PoolQueue *q, int nk, int dg) {
   int *t = &q->ticket[0][0][0];
-  for (int i = threadIdx.x; i < nk * 16; i += blockDim.x) t[i] = 0;
+  for (int i = threadIdx.x; i < nk * kPoolLists * 8; i += blockDim.x) t[i] = 0;
   if (threadIdx.x == 0) q->g += dg;
 }
 
-__global__ void __launch_bounds__(64) pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0,
-                                               int count1, int g) {
-  if (threadIdx.x == 0) {
-    q->items[0] = items0;
-    q->items[1] = items1;
-    q->count[0] = count0;
-    q->count[1] = count1;
-    if (g >= 0) q->g = g;
+struct PoolLists {
+  const int4 *items[kPoolLists];
+  int count[kPoolLists];
+};
+
+__global__ void __launch_bounds__(64) pool_set(PoolQueue *q, PoolLists l, int g) {
+  if (threadIdx.x < kPoolLists) {
+    q->items[threadIdx.x] = l.items[threadIdx.x];
+    q->count[threadIdx.x] = l.count[threadIdx.x];
   }
+  if (threadIdx.x == 0 && g >= 0) q->g = g;
 }
 
 template <int T, int D, int MODE>
@@ -56,8 +58,9 @@ hipError_t launch_pool_d(const StepArgs *descs, PoolQueue *q, int k, int tp, int
 
 hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D, int medium_kind,
                        int grid, hipStream_t stream) {
-  if (grid <= 0 || k < 0 || k >= kMaxChunk || cls < 0 || cls > 2) return hipErrorInvalidValue;
+  if (grid <= 0 || k < 0 || k >= kMaxChunk || cls < 0 || cls > 3) return hipErrorInvalidValue;
   if (cls == 1) return launch_pool_d<1, 1>(descs, q, k, tp, T, medium_kind, grid, stream);
+  if (cls == 3) return launch_pool_d<1, 3>(descs, q, k, tp, T, medium_kind, grid, stream);
   if (cls == 2) {
     if (D == 2) return launch_pool_d<2, 2>(descs, q, k, tp, T, medium_kind, grid, stream);
     return launch_pool_d<1, 2>(descs, q, k, tp, T, medium_kind, grid, stream);
@@ -66,9 +69,14 @@ hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int c
   return launch_pool_d<1, 0>(descs, q, k, tp, T, medium_kind, grid, stream);
 }
 
-hipError_t launch_pool_set(PoolQueue *q, const int4 *items0, const int4 *items1, int count0, int count1, int g,
+hipError_t launch_pool_set(PoolQueue *q, const int4 *const items[kPoolLists], const int count[kPoolLists], int g,
                            hipStream_t stream) {
-  hipLaunchKernelGGL(pool_set, dim3(1), dim3(64), 0, stream, q, items0, items1, count0, count1, g);
+  PoolLists l;
+  for (int c = 0; c < kPoolLists; ++c) {
+    l.items[c] = items[c];
+    l.count[c] = count[c];
+  }
+  hipLaunchKernelGGL(pool_set, dim3(1), dim3(64), 0, stream, q, l, g);
   return hipGetLastError();
 }
 

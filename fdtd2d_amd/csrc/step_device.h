@@ -729,38 +729,50 @@ __device__ __forceinline__ void run_tile(const StepArgs &a, const float4 *lut_s,
 }
 
 // A CPML-class entry by its kind (set by the host from the entry's window,
-// see kKindGeneral .. kKindE). The row loops of the kinds with events are
-// compiled with the RC probe and processing-line captures and without, the
-// latter for problems that have neither. FULL = false compiles only the
-// kinds of fused_kinds(MED), the event kind without captures: the fused
-// single-grid kernel, whose list holds only those (with the general kind it
-// would need 168 VGPRs and spill 274).
-template <int T, int MED, bool FULL = true>
+// see kKindGeneral .. kKindE). SET picks the kinds compiled: 0 every kind
+// (the CPML-class kernel of an unfused launch); 1 the fusable kinds
+// (fusable_kind), for the fused kernels: every kind for the density medium,
+// for the table medium the strips and the event entries of problems without
+// captures (with every kind that kernel would need 168 VGPRs and spill 96
+// bytes per lane); 2 the rest, for the kernel beside a fused one. The row
+// loops of general and event entries are compiled with the RC probe and
+// processing-line captures and without, the latter for problems that have
+// neither.
+constexpr int cpml_set(int set, int med) { return set == 1 && fused_kinds(med) == kAllKinds ? 0 : set; }
+constexpr bool set_has(int set, int med, int kind) {
+  return set == 0 ? true
+         : set == 1 ? ((fused_kinds(med) >> kind) & 1) != 0
+                    : kind == kKindGeneral || kind == kKindE || !((fused_kinds(med) >> kind) & 1);
+}
+
+template <int T, int MED, int SET_ = 0>
 __device__ __forceinline__ void run_cpml_tile(const StepArgs &a, const float4 *lut_s, int src, int n0, int x,
                                               int r0, int r1, int flags) {
-  constexpr int kinds = FULL ? 0x1f : fused_kinds(MED);
+  constexpr int SET = cpml_set(SET_, MED);
   switch (entry_kind(flags)) {
     case kKindX:
-      if constexpr ((kinds >> kKindX) & 1) run_tile<T, 1, 1, kX, MED, 1>(a, lut_s, src, n0, x, r0, r1);
+      if constexpr (set_has(SET, MED, kKindX)) run_tile<T, 1, 1, kX, MED, 1>(a, lut_s, src, n0, x, r0, r1);
       return;
     case kKindY:
-      if constexpr ((kinds >> kKindY) & 1) run_tile<T, 1, 1, kY, MED, 1>(a, lut_s, src, n0, x, r0, r1);
+      if constexpr (set_has(SET, MED, kKindY)) run_tile<T, 1, 1, kY, MED, 1>(a, lut_s, src, n0, x, r0, r1);
       return;
     case kKindXY:
-      if constexpr ((kinds >> kKindXY) & 1) run_tile<T, 1, 1, kX | kY, MED, 1>(a, lut_s, src, n0, x, r0, r1);
+      if constexpr (set_has(SET, MED, kKindXY)) run_tile<T, 1, 1, kX | kY, MED, 1>(a, lut_s, src, n0, x, r0, r1);
       return;
     case kKindE:
-      if constexpr (!((kinds >> kKindE) & 1)) return;
-      if constexpr (FULL) {
-        if (extras_on(a)) {
+      if constexpr (SET == 1) {
+        run_tile<T, 1, 1, kE, MED, 1, false>(a, lut_s, src, n0, x, r0, r1);
+      } else if constexpr (SET == 2) {
+        run_tile<T, 1, 1, kE, MED, 1, true>(a, lut_s, src, n0, x, r0, r1);
+      } else {
+        if (extras_on(a))
           run_tile<T, 1, 1, kE, MED, 1, true>(a, lut_s, src, n0, x, r0, r1);
-          return;
-        }
+        else
+          run_tile<T, 1, 1, kE, MED, 1, false>(a, lut_s, src, n0, x, r0, r1);
       }
-      run_tile<T, 1, 1, kE, MED, 1, false>(a, lut_s, src, n0, x, r0, r1);
       return;
     default:
-      if constexpr (FULL) {
+      if constexpr (set_has(SET, MED, kKindGeneral)) {
         if (extras_on(a))
           run_tile<T, 1, 1, kGeneral, MED, 1, true>(a, lut_s, src, n0, x, r0, r1);
         else
@@ -771,18 +783,23 @@ __device__ __forceinline__ void run_cpml_tile(const StepArgs &a, const float4 *l
 }
 
 // Single-grid step kernel. MODE 0: the interior list (tile_list[0]); 1: the
-// CPML-class list (tile_list[1]); 2: one list (tile_list[0]) holding the
-// interior entries and the CPML-class entries of the strip, corner and (for
-// problems with no RC probe and no processing line) event kinds, each XCD's
-// range starting with its share of the latter (flagged kFlagCpml), so that
-// one launch covers the grid; the other CPML-class entries (an event inside
-// a strip, say, or any event tile of a problem with captures; none in the
-// benchmark workloads) run in the MODE 1 kernel beside it. The fused form is
-// held to the 3 waves per SIMD the interior kernel runs at.
-// (The MODE 1 kernel, with every kind and both capture variants, takes 252
-// VGPRs, 2 waves per SIMD; held to 3 it would spill 330 VGPRs.)
+// CPML-class list (tile_list[1]), every kind; 2: one list (tile_list[0])
+// holding the interior entries and the CPML-class entries of the fusable
+// kinds, each XCD's range starting with its share of the latter (flagged
+// kFlagCpml), so that one launch covers the grid; 3: the other CPML-class
+// entries (tile_list[1]: events inside a strip's window, say, or any event
+// tile of a problem with captures; none in the 8192^2 workload), beside a
+// MODE 2 kernel. The fused form is held to the 3 waves per SIMD the interior
+// kernel runs at.
+// The arguments are read through a constant-address-space pointer to the
+// kernel argument segment (scalar loads): bound by value, the larger kernels
+// copied the whole StepArgs to scratch and read it with vector loads.
+typedef const StepArgs __attribute__((address_space(4))) *cargs_ptr;
+
 template <int T, int C, int D, int MED, int MODE>
-__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 2 ? 3 : 1) step_kernel(const StepArgs a, int src, int n0) {
+__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 2 ? 3 : 1)
+    step_kernel(const StepArgs args, int src, int n0) {
+  const StepArgs &a = *(const StepArgs *)((cargs_ptr)__builtin_amdgcn_kernarg_segment_ptr());
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   if constexpr (MED == kTable) {
     for (int q = threadIdx.x; q < a.nmat; q += blockDim.x) lut_s[q] = gload4(a.lut, q);
@@ -797,16 +814,16 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 2 ? 3 : 1) step_kern
   const int pos = xcd * q + min(xcd, rr) + (b >> 3);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long e = 4LL * (pos * kWavesPerWG + wave);
-  const int4 *list = a.tile_list[MODE == 1 ? 1 : 0];
+  const int4 *list = a.tile_list[(MODE == 1 || MODE == 3) ? 1 : 0];
   const int tx = cld_i(list, e), r0 = cld_i(list, e + 1), r1 = cld_i(list, e + 2), flags = cld_i(list, e + 3);
   if (r0 >= r1) return;  // padding entry
-  if constexpr (MODE == 1) {
-    run_cpml_tile<T, MED>(a, lut_s, src, n0, tx, r0, r1, flags);
+  if constexpr (MODE == 1 || MODE == 3) {
+    run_cpml_tile<T, MED, MODE == 1 ? 0 : 2>(a, lut_s, src, n0, tx, r0, r1, flags);
     return;
   } else {
     if constexpr (MODE == 2) {
       if (flags & kFlagCpml) {
-        run_cpml_tile<T, MED, false>(a, lut_s, src, n0, tx, r0, r1, flags);
+        run_cpml_tile<T, MED, 1>(a, lut_s, src, n0, tx, r0, r1, flags);
         return;
       }
     }
@@ -832,22 +849,25 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 2 ? 3 : 1) step_kern
 // lists are ordered shot by shot, so an XCD keeps to a few shots and their
 // halo rows stay in its L2), then steals from the other ranges. The shot's
 // StepArgs are read through the constant address space (scalar loads).
-typedef const StepArgs __attribute__((address_space(4))) *cargs_ptr;
 
-// MODE 0: interior tiles, 1: CPML-class tiles, 2: both, the CPML-class
-// ones first (one kernel for a whole launch: every wave takes CPML-class work
+// MODE 0: interior tiles (list 0); 1: CPML-class tiles of every kind (list
+// 1); 2: both, the CPML-class ones first (list 1, the fusable kinds; then
+// list 0: one kernel for a whole launch, every wave takes CPML-class work
 // while there is any, so the longer CPML-class tiles never run at the tail
-// with the rest of the device idle).
-// The CPML-class form (MODE 1) needs more than the 128 VGPRs of 4 waves per
-// SIMD at T = 8 and runs at 3 rather than spill.
+// with the rest of the device idle); 3: the leftover CPML-class tiles (list
+// 2) beside a MODE 2 kernel.
+// The interior form runs at 4 waves per SIMD (at most 128 VGPRs); the forms
+// with CPML-class tiles need more at T = 8 and run at 3 (a few VGPRs spill
+// in some); the leftover form, which carries the general kind, at 2 rather
+// than spill (its list is short).
 template <int T, int C, int D, int MED, int MODE>
-__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 1 ? 3 : 4)
+__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 0 ? 4 : MODE == 3 ? 2 : 3)
     pool_kernel(const StepArgs *__restrict__ descs, PoolQueue *q, int k, int tp) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   __shared__ int s_pos, s_cls;
   const int g = __builtin_amdgcn_readfirstlane(q->g) + k;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int x = blockIdx.x & 7, tried = 0, cls_t = MODE == 0 ? 0 : 1;  // used by thread 0 only
+  int x = blockIdx.x & 7, tried = 0, cls_t = MODE == 0 ? 0 : MODE == 3 ? 2 : 1;  // used by thread 0 only
   int lut_slot = -1;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -893,13 +913,13 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 1 ? 3 : 4)
     const int n0 = a.n_base + dg * tp;
     const int src = a.src_base ^ (dg & 1);
     if (r0 < r1) {
-      if constexpr (MODE == 1) {
-        run_cpml_tile<T, MED>(a, lut_s, src, n0, tx, r0, r1, w);
+      if constexpr (MODE == 1 || MODE == 3) {
+        run_cpml_tile<T, MED, MODE == 1 ? 0 : 2>(a, lut_s, src, n0, tx, r0, r1, w);
       } else {
         bool edge_tile = false;
         if constexpr (MODE == 2) {
           edge_tile = cls == 1;
-          if (edge_tile) run_cpml_tile<T, MED>(a, lut_s, src, n0, tx, r0, r1, w);
+          if (edge_tile) run_cpml_tile<T, MED, 1>(a, lut_s, src, n0, tx, r0, r1, w);
         }
         if (!edge_tile) {
           if (w & 1)

@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) rc_kernel(const RcArgs r, int n0, int t) 
 
 template <int T, int C, int D, int MODE>
 hipError_t launch_tcd(const StepArgs &a, int src, int n0, hipStream_t s) {
-  const int grid = a.n_wg[MODE == 1 ? 1 : 0];
+  const int grid = a.n_wg[(MODE == 1 || MODE == 3) ? 1 : 0];
   if (grid == 0) return hipSuccess;
   if (a.medium_kind == kTable)
     hipLaunchKernelGGL((step_kernel<T, C, D, kTable, MODE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, a, src, n0);
@@ -113,7 +113,8 @@ hipError_t launch_step(const StepArgs &a, int cls, int T, int C, int D, int src,
   // CPML-class work-groups always run one column per lane with the shallow
   // prefetch (their state already fills the register file); interior ones
   // take the requested shape.
-  if (cls != 0) return launch_cd<1, 1, 1>(a, T, src, n0, stream);
+  if (cls == 1) return launch_cd<1, 1, 1>(a, T, src, n0, stream);
+  if (cls == 2) return launch_cd<1, 1, 3>(a, T, src, n0, stream);
   switch (C * 16 + D) {
     case 1 * 16 + 1: return launch_cd<1, 1, 0>(a, T, src, n0, stream);
     case 1 * 16 + 2: return launch_cd<1, 2, 0>(a, T, src, n0, stream);

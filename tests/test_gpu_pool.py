@@ -62,7 +62,8 @@ def _problems(seed, n, table=False, sizes=(80, 330), nsteps=None, steps=(9, 45))
 
 
 @pytest.mark.parametrize("table,chunk,tblock,fused,depth", [(False, 4, 8, 1, 2), (True, 4, 8, 0, 1),
-                                                             (False, 0, 5, 0, 2), (True, 2, 3, 1, 1)])
+                                                             (False, 0, 5, 0, 2), (True, 2, 3, 1, 1),
+                                                             (True, 4, 8, 1, 2)])
 def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock, fused, depth):
     """Uneven shots (sizes, CPML, steps not multiples of T) in one pool."""
     probs = _problems(900 + 10 * tblock + chunk, 7, table=table, steps=(45, 90))
