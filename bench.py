@@ -196,6 +196,7 @@ def bench_batch(args) -> int:
 
     import fdtd2d_amd as F
     from fdtd2d_amd.batch import build_setups, run_pool
+    from fdtd2d_amd.tuning import ShapeCache
 
     t0 = time.time()
     setups = build_setups(3, 3, range(args.shots))
@@ -203,7 +204,9 @@ def bench_batch(args) -> int:
     cells = sum(float(p.nx) * p.ny * p.nsteps for p in probs)
     log(f"batch: {len(setups)} shots, {cells / 1e12:.3f} T cell updates, set up in {time.time() - t0:.1f}s")
 
-    pool, ids, st = run_pool(setups, 0, tune=not args.no_tune, log=log)
+    cache = None if args.shape_cache == "off" else ShapeCache()
+    pool, ids, st = run_pool(setups, 0, tune=not args.no_tune, log=log, cache=cache,
+                             lookup=args.shape_cache == "use")
     det_b = [pool.read_detectors(i)[0] for i in ids]
     pool.close()
     tb = st["seconds"]
@@ -267,6 +270,9 @@ def bench_batch(args) -> int:
                    "cell_updates": cells, "parallelism": "single GPU"},
         "batched": {"seconds": round(tb, 4), "gcell_per_s": round(cells / tb / 1e9, 2),
                     "shape": st["tuned"]["shape"] if st["tuned"] else None,
+                    "shape_source": st["tuned"]["source"] if st["tuned"] else "pool defaults",
+                    "mix_key": st["tuned"]["key"] if st["tuned"] else None,
+                    "tune_trials": len(st["tuned"]["trials"]) if st["tuned"] else 0,
                     "graph_replays": st["graph_replays"], "eager_launches": st["eager_launches"]},
         "plain": None if tp is None else {"seconds": round(tp, 4), "sims_per_s": round(len(setups) / tp, 3),
                                           "gcell_per_s": round(cells / tp / 1e9, 2)},
@@ -283,7 +289,8 @@ def bench_serve(args) -> int:
     """Served streams through the continuous-batching loop: frequency sweeps
     (--mode serve-sweep) or the mixed 1024^2 / 16384^2 stream (--mode
     serve-mixed), at --rate requests/s for --duration seconds of arrivals."""
-    from fdtd2d_amd.serve import Server, latency_summary, mixed_requests, sweep_requests
+    from fdtd2d_amd.serve import Server, latency_summary, mixed_requests, request_mix_key, sweep_requests
+    from fdtd2d_amd.tuning import ShapeCache
 
     t0 = time.time()
     if args.mode == "serve-sweep":
@@ -299,9 +306,19 @@ def bench_serve(args) -> int:
     nshots = sum(len(r.shots) for r in reqs)
     log(f"{args.mode}: {len(reqs)} requests ({nshots} shots) at {rate} req/s over {args.duration}s; "
         f"drawn in {time.time() - t0:.1f}s")
-    shape = json.loads(args.pool_shape)
-    log(f"pool shape {shape}")
-    srv = Server(0, budget_bytes=budget, **shape)
+    srv = Server(0, budget_bytes=budget)
+    key = request_mix_key(reqs)
+    if args.pool_shape:
+        shape, source = json.loads(args.pool_shape), "--pool-shape"
+    else:
+        # the shape cached for the nearest shot mix, else the batch mix's
+        shape, source = (None, None) if args.shape_cache == "off" else \
+            ShapeCache().lookup("pool", key, int(srv.pool.info("cus")), nearest=True)
+        if shape is None:
+            shape, source = dict(SERVE_SHAPE), "built-in default"
+    for name, value in shape.items():
+        srv.pool.set_option(name, value)
+    log(f"pool shape {shape} ({source}; stream mix {key})")
     done, st = srv.run(reqs, max_seconds=args.duration + args.drain, log=log)
     srv.close()
     lat = latency_summary(done, classes)
@@ -318,6 +335,9 @@ def bench_serve(args) -> int:
         "data": "synthetic",
         "config": {"workload": args.mode, "rate_req_per_s": rate, "arrival_window_s": args.duration,
                    "requests": len(reqs), "shots": nshots, "budget_bytes": budget, "parallelism": "single GPU"},
+        "pool_shape": shape,
+        "shape_source": source,
+        "mix_key": key,
         "latency_s": lat,
         "completed_requests": len(done),
         "unfinished_requests": st["unfinished_requests"],
@@ -350,9 +370,13 @@ def main() -> int:
     ap.add_argument("--drain", type=float, default=120.0)
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--shape-cache", choices=["use", "retune", "off"], default="use",
+                    help="batch and serving pools: take the launch shape cached for the shot mix (use), run the "
+                         "autotuner and cache its result (retune), or run it without the cache (off)")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--pool-shape", default=json.dumps(SERVE_SHAPE),
-                    help="launch shape of the serving pool, as JSON pool options")
+    ap.add_argument("--pool-shape", default=None,
+                    help="launch shape of the serving pool, as JSON pool options (default: the shape cached for "
+                         "the nearest shot mix, else a built-in one)")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)

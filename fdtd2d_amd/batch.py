@@ -2,12 +2,13 @@
 launch-shape autotuner.
 
 The autotuner tunes a live pool by coordinate descent over the launch shape
-(one fused kernel or two per launch, steps per launch, band rows, CPML
-sub-band rows, persistent work-groups per CU of each kernel, prefetch
-depth). Every trial advances the real shots -
+(one fused kernel or two per launch, steps per launch, columns per lane,
+band rows, CPML sub-band rows, persistent work-groups per CU of each kernel,
+prefetch depth). Every trial advances the real shots -
 results are bitwise independent of the shape - and is scored by the cell
 updates it completed per second of wall time, measured with the pool synced
-on both sides.
+on both sides. Shapes found are cached per shot mix (tuning.ShapeCache), and
+a pool whose mix is in the cache takes that shape with no trials.
 """
 
 from __future__ import annotations
@@ -16,11 +17,13 @@ import time
 from concurrent.futures import ThreadPoolExecutor
 
 from .problem import Pool, Setup
+from .tuning import ShapeCache, mix_key
 
 # coordinate-descent axes, in the order they are tuned; the first value of
 # each is the starting point (the shape that won on uneven shot mixes)
 TUNE_AXES = [
     ("tblock", [8, 6, 7]),
+    ("cols_per_lane", [1, 2]),
     ("band", [256, 128, 64]),
     ("wg_per_cu", [3, 4]),
     ("band_cpml", [32, 16, 64]),
@@ -103,14 +106,49 @@ def autotune(pool: Pool, launches_per_trial: int = 4, axes=TUNE_AXES, log=None, 
     return {"shape": best, "rate": best_rate, "trials": trials}
 
 
-def run_pool(setups, device: int = 0, tune: bool = True, chunk_launches: int = 8, log=None, **options):
+def shot_sizes(setups):
+    """(nx, ny, nsteps, medium_kind) of each setup, for mix_key."""
+    out = []
+    for s in setups:
+        p = s.c_problem.contents
+        out.append((p.nx, p.ny, p.nsteps, p.medium_kind))
+    return out
+
+
+def choose_shape(pool, key: str, cache: ShapeCache | None, log=None, lookup: bool = True) -> dict:
+    """The launch shape for the shots now in the pool: the cached one for
+    their mix when there is one (no trials; lookup=False skips the cache),
+    else the autotuner's, which is then stored. Returns the autotune record
+    with "source" (the cache file or "trials") and "key"."""
+    cus = int(pool.info("cus"))
+    if cache is not None and key and lookup:
+        shape, src = cache.lookup("pool", key, cus)
+        if shape is not None:
+            for name, value in shape.items():
+                pool.set_option(name, value)
+            if log:
+                log(f"shape for mix {key} from {src}: {shape}")
+            return {"shape": shape, "rate": None, "trials": [], "source": src, "key": key}
+    tuned = autotune(pool, log=log)
+    tuned["source"] = "trials"
+    tuned["key"] = key
+    if cache is not None and key and tuned["shape"]:
+        cache.store("pool", key, cus, tuned["shape"], tuned["rate"])
+    return tuned
+
+
+def run_pool(setups, device: int = 0, tune: bool = True, chunk_launches: int = 8, log=None,
+             cache: ShapeCache | None = None, lookup: bool = True, **options):
     """All setups through one pool to completion. Returns (pool, ids, stats);
-    the caller reads results and closes the pool."""
+    the caller reads results and closes the pool. tune: take the shape from
+    `cache` (a hit) or the autotuner (a miss, or lookup=False; stored in
+    `cache`)."""
     pool = Pool(device, **options)
     ids = [pool.add(s) for s in setups]
     pool.sync()
+    key = mix_key(shot_sizes(setups))
     t0 = time.perf_counter()
-    tuned = autotune(pool, log=log) if tune else None
+    tuned = choose_shape(pool, key, cache, log=log, lookup=lookup) if tune else None
     while pool.info("running") > 0:
         pool.step(chunk_launches)
     pool.sync()

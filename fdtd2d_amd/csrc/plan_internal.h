@@ -185,10 +185,14 @@ hipError_t launch_rc(const RcArgs &r, int n0, int t, hipStream_t stream);
 // slot << 8, one slot per work-group) from q's lists until they are
 // exhausted. cls: 0 interior (list 0), 1 CPML class (list 1, every kind),
 // 2 fused (list 1 of fusable kinds, then list 0), 3 the leftover CPML-class
-// entries (list 2). tp = the pool's steps per full launch. Only C = 1;
-// D = 1 or 2 for the interior class.
-hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D, int medium_kind,
-                       int grid, hipStream_t stream);
+// entries (list 2). tp = the pool's steps per full launch. C = 1 or 2
+// columns per lane and D = 1 or 2 rows of prefetch for the interior tiles of
+// classes 0 and 2 (CPML-class tiles run at one column per lane).
+hipError_t launch_pool(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int C, int D,
+                       int medium_kind, int grid, hipStream_t stream);
+// The C = 2 forms of classes 0 and 2 (their own translation unit).
+hipError_t launch_pool_c2(const StepArgs *descs, PoolQueue *q, int k, int tp, int cls, int T, int D,
+                          int medium_kind, int grid, hipStream_t stream);
 // Resets the tickets of launches [0, nk) and adds dg to q->g (one wave).
 hipError_t launch_pool_tick(PoolQueue *q, int nk, int dg, hipStream_t stream);
 // Points q at new lists (g >= 0 also sets q->g), in stream order.

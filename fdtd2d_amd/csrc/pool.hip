@@ -129,7 +129,7 @@ struct fdtd2d_pool {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // launch shape
   // (defaults: the shape the autotuner picks for mixed shot sizes)
-  int T = 8, band = 256, band_cpml = 32, D = 1;
+  int T = 8, band = 256, band_cpml = 32, D = 1, C = 1;
   int wg_per_cu = 3, wg_per_cu_cpml = 3;
   int fused = 0;  // 0: one kernel per tile class on two streams; 1: one kernel, CPML-class work first
   int chunk = 8;  // launches per captured graph; 0 = no graphs
@@ -142,6 +142,7 @@ struct fdtd2d_pool {
   std::vector<int> free_slots;
   // device state
   fdtd2d::StepArgs *d_descs = nullptr;
+  std::vector<fdtd2d::StepArgs> h_descs;  // host copy of the descriptor table
   size_t cap_slots = 0;
   // main lists, double-buffered: a rebuild writes the buffer the launches in
   // flight do not read, once the last launch that read it has finished
@@ -234,14 +235,29 @@ int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
   return FDTD2D_OK;
 }
 
-int upload_desc(fdtd2d_pool *pp, int slot) {
-  fdtd2d_plan *pl = pp->shots[slot].pl;
-  fdtd2d::StepArgs a = pl->a;
+// Fills the host copy of shot `slot`'s descriptor: its state at the next
+// pool launch.
+void fill_desc(fdtd2d_pool *pp, int slot) {
+  const fdtd2d_plan *pl = pp->shots[slot].pl;
+  fdtd2d::StepArgs &a = pp->h_descs[slot];
+  a = pl->a;
   a.g_base = pp->g;
   a.n_base = pl->n;
   a.src_base = pl->src;
-  HIP_TRY(hipMemcpy(pp->d_descs + slot, &a, sizeof(a), hipMemcpyHostToDevice), "hipMemcpy descriptor");
+}
+
+// Copies descriptors [lo, hi) of the host table to the device table.
+int copy_descs(fdtd2d_pool *pp, size_t lo, size_t hi) {
+  if (hi > lo)
+    HIP_TRY(hipMemcpy(pp->d_descs + lo, pp->h_descs.data() + lo, (hi - lo) * sizeof(fdtd2d::StepArgs),
+                      hipMemcpyHostToDevice),
+            "hipMemcpy descriptors");
   return FDTD2D_OK;
+}
+
+int upload_desc(fdtd2d_pool *pp, int slot) {
+  fill_desc(pp, slot);
+  return copy_descs(pp, slot, slot + 1);
 }
 
 hipEvent_t take_event(fdtd2d_pool *pp) {
@@ -290,12 +306,13 @@ int launch_main(fdtd2d_pool *pp, int k) {
   if (side) {
     HIP_TRY(hipEventRecord(pp->ev_fork, pp->stream), "hipEventRecord fork");
     HIP_TRY(hipStreamWaitEvent(pp->aux, pp->ev_fork, 0), "hipStreamWaitEvent fork");
-    e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, pp->fused ? 3 : 1, pp->T, 1, pp->medium_kind,
+    e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, pp->fused ? 3 : 1, pp->T, 1, 1, pp->medium_kind,
                             pp->fused ? std::min(gc, pp->main_count[2]) : gc, pp->aux);
     if (e != hipSuccess) return hip_fail(e, "pool CPML kernel launch");
     HIP_TRY(hipEventRecord(pp->ev_join, pp->aux), "hipEventRecord join");
   }
-  e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, pp->fused ? 2 : 0, pp->T, pp->D, pp->medium_kind, gi,
+  e = fdtd2d::launch_pool(pp->d_descs, pp->d_q, k, pp->T, pp->fused ? 2 : 0, pp->T, pp->C, pp->D, pp->medium_kind,
+                          gi,
                           pp->stream);
   if (e != hipSuccess) return hip_fail(e, "pool kernel launch");
   if (side) HIP_TRY(hipStreamWaitEvent(pp->stream, pp->ev_join, 0), "hipStreamWaitEvent join");
@@ -349,18 +366,18 @@ int run_tails(fdtd2d_pool *pp, const std::vector<int> &tails) {
       // in stream order: the leftover kernel's tiles are disjoint from the
       // fused kernel's, both read set src and write set 1 - src
       if (e == hipSuccess && n2 > 0)
-        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 3, t, 1, pp->medium_kind, std::min(gc, n2),
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 3, t, 1, 1, pp->medium_kind, std::min(gc, n2),
                                 pp->stream);
       if (e == hipSuccess && n0 + n1 > 0)
-        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 2, t, pp->D, pp->medium_kind, std::min(gi, n0 + n1),
-                                pp->stream);
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 2, t, pp->C, pp->D, pp->medium_kind,
+                                std::min(gi, n0 + n1), pp->stream);
     } else {
       if (e == hipSuccess && n1 > 0)
-        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 1, t, 1, pp->medium_kind, std::min(gc, n1),
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 1, t, 1, 1, pp->medium_kind, std::min(gc, n1),
                                 pp->stream);
       if (e == hipSuccess && n0 > 0)
-        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 0, t, pp->D, pp->medium_kind, std::min(gi, n0),
-                                pp->stream);
+        e = fdtd2d::launch_pool(pp->d_descs, pp->d_qt, 0, pp->T, 0, t, pp->C, pp->D, pp->medium_kind,
+                                std::min(gi, n0), pp->stream);
     }
     if (e == hipSuccess) e = fdtd2d::launch_pool_tick(pp->d_qt, 1, 0, pp->stream);
     if (e != hipSuccess) return hip_fail(e, "pool tail launch");
@@ -377,17 +394,22 @@ int run_tails(fdtd2d_pool *pp, const std::vector<int> &tails) {
   return FDTD2D_OK;
 }
 
+// Retiles every live shot at the pool's launch shape (no launch in flight);
+// the descriptors go to the device in one copy.
 int reshape(fdtd2d_pool *pp) {
   for (size_t s = 0; s < pp->shots.size(); ++s) {
     Shot &sh = pp->shots[s];
     if (!sh.live) continue;
     sh.pl->T = pp->T;
+    sh.pl->C = pp->C;
     sh.pl->band = pp->band;
     sh.pl->band_cpml = pp->band_cpml;
     int rc = set_tiling(sh.pl);
     if (rc) return rc;
-    if ((rc = upload_desc(pp, static_cast<int>(s)))) return rc;
+    fill_desc(pp, static_cast<int>(s));
   }
+  int rc = copy_descs(pp, 0, pp->shots.size());
+  if (rc) return rc;
   pp->lists_valid = false;
   pp->graph_valid = false;
   return FDTD2D_OK;
@@ -548,6 +570,10 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
     if (v < 0) return set_error(FDTD2D_EINVAL, "band_cpml must be >= 0");
     pp->band_cpml = static_cast<int>(v);
     shape = true;
+  } else if (n == "cols_per_lane") {
+    if (v != 1 && v != 2) return set_error(FDTD2D_EINVAL, "cols_per_lane must be 1 or 2");
+    pp->C = static_cast<int>(v);
+    shape = true;
   } else if (n == "prefetch") {
     if (v != 1 && v != 2) return set_error(FDTD2D_EINVAL, "prefetch must be 1 or 2");
     pp->D = static_cast<int>(v);
@@ -590,6 +616,7 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value)
   else if (n == "band") *value = pp->band;
   else if (n == "band_cpml") *value = pp->band_cpml;
   else if (n == "prefetch") *value = pp->D;
+  else if (n == "cols_per_lane") *value = pp->C;
   else if (n == "wg_per_cu") *value = pp->wg_per_cu;
   else if (n == "fused") *value = pp->fused;
   else if (n == "wg_per_cu_cpml") *value = pp->wg_per_cu_cpml;
@@ -625,7 +652,7 @@ int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
     if (!base) return set_error(FDTD2D_ENOMEM, "over the pool's memory budget");
   }
   fdtd2d_plan *pl = nullptr;
-  rc = create_plan(p, pp->device, true, pp->stream, pp->T, 1, pp->band, pp->band_cpml, base, &pl);
+  rc = create_plan(p, pp->device, true, pp->stream, pp->T, pp->C, pp->band, pp->band_cpml, base, &pl);
   if (rc) {
     if (base) pp->arena.give(base);
     return rc;
@@ -657,6 +684,7 @@ int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
     }
     pp->d_descs = nd;
     pp->cap_slots = nc;
+    pp->h_descs.resize(nc);
     pp->graph_valid = false;
   }
   Shot &sh = pp->shots[slot];

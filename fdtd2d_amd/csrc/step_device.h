@@ -856,12 +856,13 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 2 ? 3 : 1)
 // while there is any, so the longer CPML-class tiles never run at the tail
 // with the rest of the device idle); 3: the leftover CPML-class tiles (list
 // 2) beside a MODE 2 kernel.
-// The interior form runs at 4 waves per SIMD (at most 128 VGPRs); the forms
-// with CPML-class tiles need more at T = 8 and run at 3 (a few VGPRs spill
-// in some); the leftover form, which carries the general kind, at 2 rather
-// than spill (its list is short).
+// The interior form runs at 4 waves per SIMD (at most 128 VGPRs; at 3 with
+// two columns per lane and two rows of prefetch, which would spill 72 bytes
+// per lane at 4); the forms with CPML-class tiles need more at T = 8 and run
+// at 3 (a few VGPRs spill in some); the leftover form, which carries the
+// general kind, at 2 rather than spill (its list is short).
 template <int T, int C, int D, int MED, int MODE>
-__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 0 ? 4 : MODE == 3 ? 2 : 3)
+__global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 0 ? (C > 1 && D > 1 ? 3 : 4) : MODE == 3 ? 2 : 3)
     pool_kernel(const StepArgs *__restrict__ descs, PoolQueue *q, int k, int tp) {
   __shared__ float4 lut_s[MED == kTable ? 256 : 1];
   __shared__ int s_pos, s_cls;
@@ -932,6 +933,34 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 0 ? 4 : MODE == 3 ? 
     // every wave is done with s_pos and the LUT before they are rewritten
     __syncthreads();
   }
+}
+
+// Launches of the pool kernel for one shape.
+template <int T, int C, int D, int MODE>
+hipError_t launch_pool_t(const StepArgs *descs, PoolQueue *q, int k, int tp, int med, int grid, hipStream_t s) {
+  if (med == kTable)
+    hipLaunchKernelGGL((pool_kernel<T, C, D, kTable, MODE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q, k,
+                       tp);
+  else
+    hipLaunchKernelGGL((pool_kernel<T, C, D, kDensity, MODE>), dim3(grid), dim3(64 * kWavesPerWG), 0, s, descs, q,
+                       k, tp);
+  return hipGetLastError();
+}
+
+template <int C, int D, int MODE>
+hipError_t launch_pool_d(const StepArgs *descs, PoolQueue *q, int k, int tp, int T, int med, int grid,
+                         hipStream_t s) {
+  switch (T) {
+    case 1: return launch_pool_t<1, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 2: return launch_pool_t<2, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 3: return launch_pool_t<3, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 4: return launch_pool_t<4, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 5: return launch_pool_t<5, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 6: return launch_pool_t<6, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 7: return launch_pool_t<7, C, D, MODE>(descs, q, k, tp, med, grid, s);
+    case 8: return launch_pool_t<8, C, D, MODE>(descs, q, k, tp, med, grid, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 // Launches of the fused single-grid kernel (MODE 2) for one shape.

@@ -28,6 +28,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .problem import Pool, Setup
+from .tuning import mix_key
 
 
 @dataclass
@@ -73,6 +74,21 @@ def mixed_requests(rate: float, duration: float, seed: int = 5, steps: int = 0, 
         st = (big_steps or steps) if big else steps
         reqs.append(Request(r, t, [(5, seed, r, st)], "16384" if big else "1024"))
     return reqs
+
+
+def request_mix_key(requests, sample: int = 512) -> str:
+    """tuning.mix_key of the shots of (up to `sample` shots of) a request
+    stream, from the workload draws alone (no arrays are built)."""
+    sizes = []
+    for r in requests:
+        for w, seed, shot, steps in r.shots:
+            if len(sizes) >= sample:
+                break
+            head = Setup.workload(w, seed=seed, shot=shot, steps=-1)
+            n = int(head.get("nMax")) if steps <= 0 else steps
+            sizes.append((int(head.get("Imax")), int(head.get("Jmax")), n, 1))  # the density medium
+            head.close()
+    return mix_key(sizes)
 
 
 class Server:

@@ -61,16 +61,17 @@ def _problems(seed, n, table=False, sizes=(80, 330), nsteps=None, steps=(9, 45))
     return out
 
 
-@pytest.mark.parametrize("table,chunk,tblock,fused,depth", [(False, 4, 8, 1, 2), (True, 4, 8, 0, 1),
-                                                             (False, 0, 5, 0, 2), (True, 2, 3, 1, 1),
-                                                             (True, 4, 8, 1, 2)])
-def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock, fused, depth):
+@pytest.mark.parametrize("table,chunk,tblock,fused,depth,cols", [
+    (False, 4, 8, 1, 2, 1), (True, 4, 8, 0, 1, 1), (False, 0, 5, 0, 2, 1), (True, 2, 3, 1, 1, 1),
+    (True, 4, 8, 1, 2, 1), (False, 4, 8, 1, 2, 2), (True, 2, 8, 0, 1, 2), (False, 0, 6, 0, 2, 2),
+    (True, 4, 7, 1, 2, 2)])
+def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock, fused, depth, cols):
     """Uneven shots (sizes, CPML, steps not multiples of T) in one pool."""
-    probs = _problems(900 + 10 * tblock + chunk, 7, table=table, steps=(45, 90))
+    probs = _problems(900 + 10 * tblock + chunk + 1000 * (cols - 1), 7, table=table, steps=(45, 90))
     with F.Pool(gpu_device, tblock=tblock, graph_chunk=chunk, fused=fused, prefetch=depth, band=0,
-                band_cpml=0) as pool:
+                band_cpml=0, cols_per_lane=cols) as pool:
         ids = [pool.add(p) for p in probs]
-        print(f"table={table} chunk={chunk} T={tblock} fused={fused} D={depth}: shots "
+        print(f"table={table} chunk={chunk} T={tblock} fused={fused} D={depth} C={cols}: shots "
               f"{[(p.nx, p.ny, p.npml, p.nsteps) for p in probs]}", flush=True)
         _drain(pool)
         print({k: pool.info(k) for k in ("launches", "graph_replays", "eager_launches", "tail_launches")})
@@ -83,12 +84,14 @@ def test_pool_shots_match_oracle(gpu_device, table, chunk, tblock, fused, depth)
 
 
 def test_pool_reshape_midrun_and_late_admission(gpu_device):
-    """tblock changed between launches, and shots admitted while others run."""
+    """tblock and columns per lane changed between launches, and shots
+    admitted while others run."""
     probs = _problems(4242, 6, table=False, nsteps=40)
     with F.Pool(gpu_device, tblock=4, graph_chunk=2) as pool:
         ids = [pool.add(p) for p in probs[:3]]
         pool.step(2)
         pool.set_option("tblock", 7)
+        pool.set_option("cols_per_lane", 2)
         ids += [pool.add(p) for p in probs[3:5]]
         pool.step(3)
         pool.set_option("tblock", 2)
