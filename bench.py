@@ -34,6 +34,10 @@ Profiling helpers (no bench line):
                         into the traffic file and a text summary on stdout
   --stats-summary KERNEL_STATS_CSV
                         print a `rocprofv3 --stats` kernel table as text
+  --sq-summary CSV [CSV ...]
+                        per-dispatch means of the SQ_* / GRBM_* counters of
+                        `rocprofv3 --pmc` runs for the single-grid step
+                        kernel, with the VALU issue fraction derived
 """
 
 from __future__ import annotations
@@ -175,6 +179,46 @@ def pmc_summary(paths: list, copy_mib: int, traffic_file: str) -> int:
     return 0
 
 
+def sq_summary(paths: list, simds: int = 1024, xcds: int = 8) -> int:
+    """SQ_* and GRBM_* counter means per dispatch of the single-grid step
+    kernel (mode 0 or 2) from one or more counter_collection.csv files, and
+    derived figures: VALU issue = SQ_INSTS_VALU over the one wave64 VALU
+    instruction per SIMD every 4 cycles that `simds` SIMDs can issue in
+    GRBM_GUI_ACTIVE / xcds cycles (GRBM_GUI_ACTIVE counts on every XCD);
+    the fraction of wave time waiting (SQ_WAIT_ANY, SQ_WAIT_INST_ANY over
+    SQ_WAVE_CYCLES, all in the same 4-cycle units)."""
+    import csv
+
+    acc: dict = {}
+    kname = None
+    for path in paths:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if _step_kernel_mode(row["Kernel_Name"]) not in (0, 2):
+                    continue
+                kname = row["Kernel_Name"]
+                v = acc.setdefault(row["Counter_Name"], [0.0, 0])
+                v[0] += float(row["Counter_Value"])
+                v[1] += 1
+    if not acc:
+        raise SystemExit("no step kernel dispatch in the counter files")
+    mean = {k: v[0] / v[1] for k, v in acc.items()}
+    print(f"kernel: {kname}")
+    for k in sorted(mean):
+        print(f"  {k:<22} {mean[k]:.4g}  ({acc[k][1]} dispatches)")
+    if "GRBM_GUI_ACTIVE" in mean:
+        cyc = mean["GRBM_GUI_ACTIVE"] / xcds
+        print(f"  cycles per XCD        {cyc:.4g}")
+        if "SQ_INSTS_VALU" in mean:
+            print(f"  VALU issue            {mean['SQ_INSTS_VALU'] / (simds * cyc / 4):.3f} "
+                  f"(of {simds} SIMDs x 1 per 4 cycles)")
+    if "SQ_WAVE_CYCLES" in mean:
+        for w in ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY"):
+            if w in mean:
+                print(f"  {w} / SQ_WAVE_CYCLES {mean[w] / mean['SQ_WAVE_CYCLES']:.3f}")
+    return 0
+
+
 def stats_summary(path: str) -> int:
     import csv
 
@@ -206,7 +250,7 @@ def bench_batch(args) -> int:
 
     cache = None if args.shape_cache == "off" else ShapeCache()
     pool, ids, st = run_pool(setups, 0, tune=not args.no_tune, log=log, cache=cache,
-                             lookup=args.shape_cache == "use")
+                             lookup=args.shape_cache == "use", warmup=True)
     det_b = [pool.read_detectors(i)[0] for i in ids]
     pool.close()
     tb = st["seconds"]
@@ -273,6 +317,7 @@ def bench_batch(args) -> int:
                     "shape_source": st["tuned"]["source"] if st["tuned"] else "pool defaults",
                     "mix_key": st["tuned"]["key"] if st["tuned"] else None,
                     "tune_trials": len(st["tuned"]["trials"]) if st["tuned"] else 0,
+                    "warmup": "shot 0 alone for 16 launches at the same shape, untimed" if st["warmup"] else None,
                     "graph_replays": st["graph_replays"], "eager_launches": st["eager_launches"]},
         "plain": None if tp is None else {"seconds": round(tp, 4), "sims_per_s": round(len(setups) / tp, 3),
                                           "gcell_per_s": round(cells / tp / 1e9, 2)},
@@ -356,6 +401,7 @@ def main() -> int:
     ap.add_argument("--pmc-summary", nargs=4, default=None)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--stats-summary", default=None)
+    ap.add_argument("--sq-summary", nargs="+", default=None)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
@@ -384,6 +430,8 @@ def main() -> int:
         return pmc_summary(args.pmc_summary, args.copy_mib, args.traffic_file)
     if args.stats_summary:
         return stats_summary(args.stats_summary)
+    if args.sq_summary:
+        return sq_summary(args.sq_summary)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:

@@ -115,45 +115,75 @@ def shot_sizes(setups):
     return out
 
 
+def cached_shape(pool, key: str, cache: ShapeCache | None, log=None) -> dict | None:
+    """Sets the shape cached for the mix `key` on the pool (best before its
+    shots are added: a shape change retiles every shot) and returns its
+    record, or None on a miss."""
+    if cache is None or not key:
+        return None
+    shape, src = cache.lookup("pool", key, int(pool.info("cus")))
+    if shape is None:
+        return None
+    for name, value in shape.items():
+        pool.set_option(name, value)
+    if log:
+        log(f"shape for mix {key} from {src}: {shape}")
+    return {"shape": shape, "rate": None, "trials": [], "source": src, "key": key}
+
+
 def choose_shape(pool, key: str, cache: ShapeCache | None, log=None, lookup: bool = True) -> dict:
     """The launch shape for the shots now in the pool: the cached one for
     their mix when there is one (no trials; lookup=False skips the cache),
     else the autotuner's, which is then stored. Returns the autotune record
     with "source" (the cache file or "trials") and "key"."""
-    cus = int(pool.info("cus"))
-    if cache is not None and key and lookup:
-        shape, src = cache.lookup("pool", key, cus)
-        if shape is not None:
-            for name, value in shape.items():
-                pool.set_option(name, value)
-            if log:
-                log(f"shape for mix {key} from {src}: {shape}")
-            return {"shape": shape, "rate": None, "trials": [], "source": src, "key": key}
+    hit = cached_shape(pool, key, cache, log) if lookup else None
+    if hit is not None:
+        return hit
     tuned = autotune(pool, log=log)
     tuned["source"] = "trials"
     tuned["key"] = key
     if cache is not None and key and tuned["shape"]:
-        cache.store("pool", key, cus, tuned["shape"], tuned["rate"])
+        cache.store("pool", key, int(pool.info("cus")), tuned["shape"], tuned["rate"])
     return tuned
 
 
+def warm_up(setup, device: int, shape: dict, launches: int = 16, **options) -> None:
+    """Runs one shot for a few launches in a pool of its own at `shape`, so
+    that the kernels' code objects are loaded and a graph has been captured
+    before a timed run."""
+    with Pool(device, **options) as pool:
+        for name, value in shape.items():
+            pool.set_option(name, value)
+        pool.add(setup)
+        pool.step(launches)
+        pool.sync()
+
+
 def run_pool(setups, device: int = 0, tune: bool = True, chunk_launches: int = 8, log=None,
-             cache: ShapeCache | None = None, lookup: bool = True, **options):
+             cache: ShapeCache | None = None, lookup: bool = True, warmup: bool = False, **options):
     """All setups through one pool to completion. Returns (pool, ids, stats);
     the caller reads results and closes the pool. tune: take the shape from
-    `cache` (a hit) or the autotuner (a miss, or lookup=False; stored in
-    `cache`)."""
+    `cache` (a hit, set before the shots are added) or the autotuner (a
+    miss, or lookup=False; stored in `cache`). warmup: with a cached shape,
+    run the first setup briefly in a pool of its own first (untimed). The
+    time in stats["seconds"] runs from the shots being resident to the last
+    launch's end, trials included."""
     pool = Pool(device, **options)
+    key = mix_key(shot_sizes(setups))
+    tuned = cached_shape(pool, key, cache, log) if tune and lookup else None
+    if warmup and tuned is not None and setups:
+        warm_up(setups[0], device, tuned["shape"], **options)
     ids = [pool.add(s) for s in setups]
     pool.sync()
-    key = mix_key(shot_sizes(setups))
     t0 = time.perf_counter()
-    tuned = choose_shape(pool, key, cache, log=log, lookup=lookup) if tune else None
+    if tune and tuned is None:
+        tuned = choose_shape(pool, key, cache, log=log, lookup=False)
     while pool.info("running") > 0:
         pool.step(chunk_launches)
     pool.sync()
     dt = time.perf_counter() - t0
     stats = {"seconds": dt, "cell_updates": pool.info("cell_updates"), "tuned": tuned,
              "launches": pool.info("launches"), "graph_replays": pool.info("graph_replays"),
-             "eager_launches": pool.info("eager_launches"), "tail_launches": pool.info("tail_launches")}
+             "eager_launches": pool.info("eager_launches"), "tail_launches": pool.info("tail_launches"),
+             "warmup": bool(warmup and tuned is not None and tuned["source"] != "trials")}
     return pool, ids, stats
