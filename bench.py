@@ -34,6 +34,11 @@ Profiling helpers (no bench line):
                         into the traffic file and a text summary on stdout
   --stats-summary KERNEL_STATS_CSV
                         print a `rocprofv3 --stats` kernel table as text
+  --pmc-total FETCH_CSV WRITE_CSV COPY_FETCH_CSV COPY_WRITE_CSV
+                        HBM bytes summed over every pool and step kernel
+                        dispatch of two counter runs (a serving window, say),
+                        calibrated like --pmc-summary, over the summed
+                        durations of those dispatches
   --sq-summary CSV [CSV ...]
                         per-dispatch means of the SQ_* / GRBM_* counters of
                         `rocprofv3 --pmc` runs for the single-grid step
@@ -176,6 +181,49 @@ def pmc_summary(paths: list, copy_mib: int, traffic_file: str) -> int:
     print(f"WRITE_SIZE {wkib:.1f} KiB/dispatch x {wscale:.3f} = {wb / 1e6:.2f} MB")
     print(f"HBM traffic per launch: {(fb + wb) / 1e6:.2f} MB")
     print(f"calibration: {cname} copying {copy_bytes} B reads FETCH {cf:.1f} KiB, WRITE {cw:.1f} KiB")
+    return 0
+
+
+def pmc_total(paths: list, copy_mib: int) -> int:
+    """Sum of calibrated FETCH_SIZE and WRITE_SIZE bytes over every dispatch
+    of the stepper's kernels, per kernel and in total, with the summed
+    dispatch durations (under --pmc rocprofv3 runs dispatches one at a time,
+    so the sum is the time the device spent in them)."""
+    import csv
+
+    def per_dispatch(path, counter):
+        out = {}
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                name = row["Kernel_Name"]
+                if "fdtd2d::" not in name:
+                    continue
+                short = name.split("(anonymous namespace)::")[-1].split("(")[0]
+                v = out.setdefault(short, [0.0, 0, 0])
+                v[0] += float(row["Counter_Value"])
+                v[1] += 1
+                v[2] += int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+        return out
+
+    copy_bytes = copy_mib * (1 << 20)
+    _, cf = _pick(_read_counter(paths[2], "FETCH_SIZE"), lambda n: "copy" in n.lower() or "elementwise" in n.lower())
+    _, cw = _pick(_read_counter(paths[3], "WRITE_SIZE"), lambda n: "copy" in n.lower() or "elementwise" in n.lower())
+    fscale, wscale = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    fetch, write = per_dispatch(paths[0], "FETCH_SIZE"), per_dispatch(paths[1], "WRITE_SIZE")
+    tot_b = tot_ns = 0.0
+    print(f"{'dispatches':>10} {'fetch GB':>9} {'write GB':>9} {'ms':>9} {'GB/s':>7}  kernel")
+    for k in sorted(set(fetch) | set(write)):
+        fb = fetch.get(k, [0, 0, 0])[0] * 1024 * fscale
+        wb = write.get(k, [0, 0, 0])[0] * 1024 * wscale
+        n = max(fetch.get(k, [0, 0, 0])[1], write.get(k, [0, 0, 0])[1])
+        ns = (fetch.get(k, [0, 0, 0])[2] + write.get(k, [0, 0, 0])[2]) / 2
+        tot_b += fb + wb
+        tot_ns += ns
+        print(f"{n:>10} {fb / 1e9:>9.2f} {wb / 1e9:>9.2f} {ns / 1e6:>9.1f} {(fb + wb) / max(ns, 1):>7.0f}  {k}")
+    print(f"total: {tot_b / 1e9:.2f} GB in {tot_ns / 1e6:.1f} ms of dispatches: {tot_b / max(tot_ns, 1):.0f} GB/s "
+          f"(calibration: {fscale:.3f} x FETCH_SIZE, {wscale:.3f} x WRITE_SIZE)")
     return 0
 
 
@@ -402,6 +450,7 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--stats-summary", default=None)
     ap.add_argument("--sq-summary", nargs="+", default=None)
+    ap.add_argument("--pmc-total", nargs=4, default=None)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
@@ -416,6 +465,10 @@ def main() -> int:
     ap.add_argument("--drain", type=float, default=120.0)
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--plan-shape", choices=["cache", "tune", "defaults"], default="cache",
+                    help="single grid: take the launch shape cached for the grid (cache; the library's defaults "
+                         "on a miss), tune it on the live plan on a miss and cache it (tune), or keep the "
+                         "library's defaults (defaults)")
     ap.add_argument("--shape-cache", choices=["use", "retune", "off"], default="use",
                     help="batch and serving pools: take the launch shape cached for the shot mix (use), run the "
                          "autotuner and cache its result (retune), or run it without the cache (off)")
@@ -432,6 +485,8 @@ def main() -> int:
         return stats_summary(args.stats_summary)
     if args.sq_summary:
         return sq_summary(args.sq_summary)
+    if args.pmc_total:
+        return pmc_total(args.pmc_total, args.copy_mib)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -445,15 +500,22 @@ def main() -> int:
         return bench_serve(args)
 
     import fdtd2d_amd as F
+    from fdtd2d_amd.tuning import ShapeCache, plan_key, plan_shape
 
     total = args.warmup + args.steps
+    # plan trials (--plan-shape tune on a cache miss) advance the real grid
+    trial_steps = 1600 if args.plan_shape == "tune" else 0
     t0 = time.time()
-    setup = F.Setup.workload(2, seed=args.seed, steps=max(total, args.cpu_steps))
+    setup = F.Setup.workload(2, seed=args.seed, steps=max(total + trial_steps, args.cpu_steps))
     p = setup.c_problem.contents
     nx, ny = p.nx, p.ny
     log(f"setup {nx}x{ny} P={p.npml} nsrc={p.nsrc} ndet={p.ndet} in {time.time() - t0:.1f}s")
 
     plan = F.Plan(setup, device=0, tblock=args.tblock)
+    shape_rec = plan_shape(plan, plan_key(nx, ny, p.npml, p.medium_kind), int(plan.info("cus")),
+                           None if args.plan_shape == "defaults" else ShapeCache(), tune=args.plan_shape == "tune",
+                           log=log)
+    log(f"plan shape {shape_rec['shape']} ({shape_rec['source']}, key {shape_rec['key']})")
     T = int(plan.info("tblock"))
     log(f"plan: pitch={plan.info('pitch'):.0f} tblock={T} band={plan.info('band'):.0f} "
         f"workgroups={plan.info('workgroups'):.0f} bytes={plan.info('bytes') / 1e9:.2f} GB")
@@ -473,6 +535,7 @@ def main() -> int:
     cells_int = plan.info("cells_interior") + (cells_cpml if fused else 0)
     plan_c = int(plan.info("cols_per_lane"))
     plan_d = int(plan.info("prefetch"))
+    band = int(plan.info("band"))
     kernel_sig = f"step_kernel<{T}, {plan_c}, {plan_d}, 1, {2 if fused else 0}>"
     det, _ = plan.read_detectors()
     plan.close()
@@ -552,6 +615,8 @@ def main() -> int:
             "nx": nx,
             "ny": ny,
             "steps_per_launch": T,
+            "launch_shape": {"cols_per_lane": plan_c, "band": band, "prefetch": plan_d, "fused": fused},
+            "shape_source": shape_rec["source"],
             "parallelism": "single GPU",
         },
         "roofline": {

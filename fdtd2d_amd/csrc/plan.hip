@@ -1204,7 +1204,11 @@ int fdtd2d_plan_get_info(const fdtd2d_plan *pl, const char *name, double *value)
   else if (n == "prefetch") *value = pl->D;
   else if (n == "jz_first") *value = pl->jz_first;
   else if (n == "fused") *value = fused_launch(pl) ? 1 : 0;
-  else return set_error(FDTD2D_EINVAL, "unknown info name " + n);
+  else if (n == "cus") {
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, pl->device), "hipDeviceGetAttribute");
+    *value = ncu;
+  } else return set_error(FDTD2D_EINVAL, "unknown info name " + n);
   return FDTD2D_OK;
 }
 

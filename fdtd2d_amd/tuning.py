@@ -13,6 +13,10 @@ the user cache only. An exact key is a hit; ``nearest`` also accepts the
 entry of the same medium and device whose bucket shares are closest (L1
 distance at most ``max_distance`` tenths), for streams whose mix is only
 known from a sample.
+
+Single plans are keyed the same way by their own grid (plan_key: the size
+bucket of each side, the CPML thickness and the medium), under kind "plan";
+tune_plan finds a plan's shape by coordinate descent on the live plan.
 """
 
 from __future__ import annotations
@@ -22,6 +26,8 @@ import os
 import tempfile
 import time
 from pathlib import Path
+
+from ._lib import FDTD2DError
 
 SHIPPED = Path(__file__).with_name("tuned_shapes.json")
 
@@ -124,3 +130,84 @@ class ShapeCache:
         except OSError:
             return False
         return True
+
+
+# Plans: the launch shape of one grid, keyed by its size buckets, CPML
+# thickness and medium. Axes of the plan descent, first value the start.
+PLAN_AXES = [
+    ("cols_per_lane", [2, 1]),
+    ("band", [96, 64, 128, 192, 256]),
+    ("prefetch", [2, 1]),
+]
+
+
+def plan_key(nx: int, ny: int, npml: int, medium: int) -> str:
+    """Key of a single grid: medium, log2 of each side rounded up, CPML."""
+    return f"m{int(medium)}|{size_bucket(nx, 2)}x{size_bucket(ny, 2)}|P{int(npml)}"
+
+
+def _plan_trial(plan, launches: int) -> float:
+    """Cell updates per second of `launches` full launches of the live plan,
+    synced on both sides; 0 when the problem has too few steps left."""
+    t = int(plan.info("tblock"))
+    steps = launches * t
+    if int(plan.info("steps")) + steps > plan.nsteps:
+        return 0.0
+    plan.sync()
+    t0 = time.perf_counter()
+    plan.run(steps)
+    plan.sync()
+    dt = time.perf_counter() - t0
+    return float(plan.nx) * plan.ny * steps / dt if dt > 0 else 0.0
+
+
+def tune_plan(plan, launches: int = 4, axes=PLAN_AXES, log=None) -> dict:
+    """Coordinate descent over the plan's launch shape (columns per lane,
+    band rows, prefetch depth) on the live plan: every trial advances the
+    real problem (results do not depend on the shape) and is scored by cell
+    updates per second. One untimed trial first (cold caches). Stops when the
+    problem runs out of steps. Leaves the plan at the best shape."""
+    start = {name: values[0] for name, values in axes}
+    best = dict(start)
+    for name, value in best.items():
+        plan.set_option(name, value)
+    _plan_trial(plan, launches)
+    best_rate = _plan_trial(plan, launches)
+    trials = [(dict(best), best_rate)]
+    for name, values in axes:
+        for v in values:
+            if v == best[name]:
+                continue
+            try:
+                plan.set_option(name, v)
+            except FDTD2DError:  # a shape this grid cannot take; the plan keeps its shape
+                continue
+            rate = _plan_trial(plan, launches)
+            if rate == 0.0:
+                break
+            trials.append(({**best, name: v}, rate))
+            if log:
+                log(f"plan tune {name}={v}: {rate / 1e9:.1f} Gcell/s (best {best_rate / 1e9:.1f})")
+            if rate > best_rate:
+                best_rate, best[name] = rate, v
+        plan.set_option(name, best[name])
+    return {"shape": best, "rate": best_rate, "trials": trials}
+
+
+def plan_shape(plan, key: str, cus: int, cache: "ShapeCache | None", tune: bool = False, log=None) -> dict:
+    """Sets the plan's launch shape from the cache entry for `key` (kind
+    "plan"); on a miss, with tune, runs tune_plan on the plan and stores the
+    result, else leaves the library's defaults. Returns {"shape", "source",
+    "key", "trials"}."""
+    if cache is not None:
+        shape, src = cache.lookup("plan", key, cus)
+        if shape is not None:
+            for name, value in shape.items():
+                plan.set_option(name, value)
+            return {"shape": shape, "source": src, "key": key, "trials": []}
+    if not tune:
+        return {"shape": None, "source": "library defaults", "key": key, "trials": []}
+    rec = tune_plan(plan, log=log)
+    if cache is not None and rec["rate"] > 0:
+        cache.store("plan", key, cus, rec["shape"], rec["rate"])
+    return {"shape": rec["shape"], "source": "trials", "key": key, "trials": rec["trials"]}

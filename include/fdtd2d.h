@@ -337,7 +337,7 @@ int fdtd2d_plan_set_option(fdtd2d_plan *plan, const char *name, double value);
    tile that owns them: x-strip tiles carry only the x-strip psi and column
    tables, y-strip tiles only the y-strip psi and row records, corner tiles
    both, event tiles only the sources, detectors and captures, general tiles
-   all of these). */
+   all of these), "cus" (compute units of the plan's device). */
 int fdtd2d_plan_get_info(const fdtd2d_plan *plan, const char *name, double *value);
 /* Advance nsteps steps (asynchronous on the plan's stream). */
 int fdtd2d_run(fdtd2d_plan *plan, int nsteps);

@@ -141,3 +141,74 @@ def test_batch_mix_has_a_shipped_shape(nearest):
     key = mix_key(sizes)
     shape, src = ShapeCache("/nonexistent/shapes.json").lookup("pool", key, 256, nearest=nearest)
     assert shape is not None and src.startswith("shipped:"), key
+
+
+class FakePlan:
+    """Enough of fdtd2d_amd.Plan for tune_plan/plan_shape: options, a step
+    counter and a step budget."""
+
+    def __init__(self, nsteps=4000, refuse=()):
+        self.options = {"tblock": 8}
+        self.n = 0
+        self.nsteps = nsteps
+        self.nx = self.ny = 1000
+        self.refuse = set(refuse)
+
+    def set_option(self, name, value):
+        from fdtd2d_amd._lib import FDTD2DError
+
+        if (name, value) in self.refuse:
+            raise FDTD2DError(-1, "fdtd2d_plan_set_option", "refused")
+        self.options[name] = value
+
+    def info(self, name):
+        return {"tblock": 8, "steps": self.n}[name]
+
+    def run(self, steps):
+        self.n += steps
+
+    def sync(self):
+        pass
+
+
+def test_plan_tuner_descends_and_caches(tmp_path, monkeypatch):
+    from fdtd2d_amd import tuning
+
+    def score(plan, launches):
+        if plan.n + 8 * launches > plan.nsteps:
+            return 0.0
+        plan.run(8 * launches)
+        o = plan.options
+        return (1000 + 20 * (o["cols_per_lane"] == 1) - abs(o["band"] - 192) / 8 + 3 * (o["prefetch"] == 1)) * 1e9
+
+    monkeypatch.setattr(tuning, "_plan_trial", score)
+    key = tuning.plan_key(8192, 8192, 16, 1)
+    assert key == "m1|13x13|P16"
+    cache = ShapeCache(tmp_path / "shapes.json", shipped=None)
+    plan = FakePlan(refuse={("band", 256)})
+    rec = tuning.plan_shape(plan, key, 256, cache, tune=True)
+    assert rec["source"] == "trials" and plan.n > 0
+    assert rec["shape"] == {"cols_per_lane": 1, "band": 192, "prefetch": 1}
+    assert plan.options["band"] == 192 and plan.options["cols_per_lane"] == 1
+    # cached: no trials
+    again = FakePlan()
+    hit = tuning.plan_shape(again, key, 256, cache, tune=True)
+    assert again.n == 0 and hit["shape"] == rec["shape"] and hit["trials"] == []
+    # no tune on a miss: the library's defaults, nothing run
+    other = FakePlan()
+    miss = tuning.plan_shape(other, "m1|9x9|P12", 256, cache, tune=False)
+    assert miss["shape"] is None and miss["source"] == "library defaults" and other.n == 0
+    # a problem that runs out of steps stops the descent
+    short = FakePlan(nsteps=8 * 4 * 3)
+    tuning.tune_plan(short)
+    assert short.n <= short.nsteps
+
+
+def test_shipped_plan_shapes_for_the_single_grid_workloads():
+    from fdtd2d_amd.tuning import plan_key
+
+    c = ShapeCache("/nonexistent/shapes.json")
+    for n in (8192, 16384):
+        shape, src = c.lookup("plan", plan_key(n, n, 16, 1), 256)
+        assert shape is not None and src.startswith("shipped:")
+        assert shape["cols_per_lane"] == 2 and shape["tblock"] == 8
