@@ -55,10 +55,10 @@ import time
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
-# launch shape of the serving pool (pool options); the serving streams mix
-# shot sizes like the batch mix, whose autotuned shape it is
-SERVE_SHAPE = {"fused": 0, "tblock": 8, "band": 256, "wg_per_cu": 3, "band_cpml": 32, "wg_per_cu_cpml": 3,
-               "prefetch": 1}
+# launch shape of a serving pool whose stream has no cached shape near its
+# mix (pool options): the pool's defaults, the batch mix's tuned shape
+SERVE_SHAPE = {"fused": 0, "tblock": 8, "cols_per_lane": 1, "band": 128, "wg_per_cu": 3, "band_cpml": 64,
+               "wg_per_cu_cpml": 2, "prefetch": 1}
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0

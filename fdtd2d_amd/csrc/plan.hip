@@ -212,8 +212,19 @@ Tiling make_tiling(int nx, int ny, int P, int T, int C, int band, int band_cpml,
   // boundaries (vw is a multiple of 16 at 8 steps per launch): at 8192^2 the
   // interior kernel alone measured 0.667 ms per launch with x0 = 56 (rows
   // starting 32 bytes into a 64-byte segment) against 0.608 with x0 = 48.
+  // Of the x0 that keep the left margin one piece (at most we columns),
+  // the one that leaves the fewest right-margin pieces (a piece costs a
+  // whole wave per row, whatever its width); ties to the smallest x0.
+  auto right_pieces = [&](int x0, int ntx) {
+    const int m = ntx > 0 ? nx - (x0 + ntx * t.vw) : nx;
+    return m <= we ? 1 : 1 + (m - we + t.vw_e - 1) / t.vw_e;
+  };
   t.x0 = (Pe + t.halo + 1 + 15) / 16 * 16;
   t.ntx = std::max(0, (nx - Pe - 2 - t.halo - t.x0) / t.vw);
+  for (int x0 = t.x0 + 16; x0 <= we && t.ntx > 0; x0 += 16) {
+    const int ntx = std::max(0, (nx - Pe - 2 - t.halo - x0) / t.vw);
+    if (ntx == t.ntx && right_pieces(x0, ntx) < right_pieces(t.x0, t.ntx)) t.x0 = x0;
+  }
   const int xr = t.ntx > 0 ? t.x0 + t.ntx * t.vw : 0;  // right margin [xr, nx)
   t.H = band > 0 ? std::max(band, T) : pick_band(nx, ny, t.vw, T, C);
   t.nty = (ny + t.H - 1) / t.H;

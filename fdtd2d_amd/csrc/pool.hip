@@ -129,8 +129,8 @@ struct fdtd2d_pool {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // launch shape
   // (defaults: the shape the autotuner picks for mixed shot sizes)
-  int T = 8, band = 256, band_cpml = 32, D = 1, C = 1;
-  int wg_per_cu = 3, wg_per_cu_cpml = 3;
+  int T = 8, band = 128, band_cpml = 64, D = 1, C = 1;
+  int wg_per_cu = 3, wg_per_cu_cpml = 2;
   int fused = 0;  // 0: one kernel per tile class on two streams; 1: one kernel, CPML-class work first
   int chunk = 8;  // launches per captured graph; 0 = no graphs
   int medium_kind = -1;

@@ -409,14 +409,14 @@ int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *cou
  * ends inside them). Shots can be added and removed between steps.
  *
  * Options (fdtd2d_pool_set_option): "tblock" (1..8, default 8), "band"
- * (default 256, 0 = per shot), "band_cpml" (default 32, 0 = per shot),
+ * (default 128, 0 = per shot), "band_cpml" (default 64, 0 = per shot),
  * "prefetch" (1, the default, or 2), "cols_per_lane" (1, the default, or 2:
  * columns per lane of the interior tiles), "fused" (0, the default: one
  * persistent kernel per tile class on two streams; 1: one kernel per launch
  * that takes the CPML-class tiles first, then the interior ones, with the
  * CPML-class kinds it does not carry in a second kernel beside it), "wg_per_cu" /
  * "wg_per_cu_cpml" (persistent work-groups per CU of the interior or fused
- * kernel / of the CPML-class kernel, default 3 / 3), "graph_chunk" (0..16,
+ * kernel / of the CPML-class kernel, default 3 / 2), "graph_chunk" (0..16,
  * 0 = no graphs),
  * "budget_bytes" (before any shot: carve shots from one arena of that size
  * and refuse shots that do not fit, FDTD2D_ENOMEM).
