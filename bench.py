@@ -227,33 +227,48 @@ def pmc_total(paths: list, copy_mib: int) -> int:
     return 0
 
 
-def sq_summary(paths: list, simds: int = 1024, xcds: int = 8) -> int:
+def sq_summary(paths: list, simds: int = 1024, xcds: int = 8, match: str | None = None) -> int:
     """SQ_* and GRBM_* counter means per dispatch of the single-grid step
-    kernel (mode 0 or 2) from one or more counter_collection.csv files, and
-    derived figures: VALU issue = SQ_INSTS_VALU over the one wave64 VALU
-    instruction per SIMD every 4 cycles that `simds` SIMDs can issue in
-    GRBM_GUI_ACTIVE / xcds cycles (GRBM_GUI_ACTIVE counts on every XCD);
-    the fraction of wave time waiting (SQ_WAIT_ANY, SQ_WAIT_INST_ANY over
-    SQ_WAVE_CYCLES, all in the same 4-cycle units)."""
+    kernel (mode 0 or 2; or of each kernel whose name contains `match`) from
+    one or more counter_collection.csv files, and derived figures: VALU
+    issue = SQ_INSTS_VALU over the one wave64 VALU instruction per SIMD every
+    4 cycles that `simds` SIMDs can issue in GRBM_GUI_ACTIVE / xcds cycles
+    (GRBM_GUI_ACTIVE counts on every XCD); the fraction of wave time waiting
+    (SQ_WAIT_ANY, SQ_WAIT_INST_ANY over SQ_WAVE_CYCLES, all in the same
+    4-cycle units)."""
     import csv
 
-    acc: dict = {}
-    kname = None
+    per: dict = {}
     for path in paths:
         with open(path, newline="") as f:
             for row in csv.DictReader(f):
-                if _step_kernel_mode(row["Kernel_Name"]) not in (0, 2):
+                name = row["Kernel_Name"]
+                if match is None:
+                    if _step_kernel_mode(name) not in (0, 2):
+                        continue
+                elif match not in name:
                     continue
-                kname = row["Kernel_Name"]
-                v = acc.setdefault(row["Counter_Name"], [0.0, 0])
+                acc = per.setdefault(name, {})
+                v = acc.setdefault(row["Counter_Name"], [0.0, 0, 0])
                 v[0] += float(row["Counter_Value"])
                 v[1] += 1
-    if not acc:
-        raise SystemExit("no step kernel dispatch in the counter files")
+                v[2] += int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+    if not per:
+        raise SystemExit("no matching kernel dispatch in the counter files")
+    for kname in sorted(per):
+        _sq_print(kname, per[kname], simds, xcds)
+    return 0
+
+
+def _sq_print(kname: str, acc: dict, simds: int, xcds: int) -> None:
     mean = {k: v[0] / v[1] for k, v in acc.items()}
+    total = {k: v[0] for k, v in acc.items()}
+    n = max(v[1] for v in acc.values())
+    dur_ms = max(v[2] for v in acc.values()) / max(n, 1) / 1e6
     print(f"kernel: {kname}")
+    print(f"  dispatches {n}, mean duration {dur_ms:.4f} ms")
     for k in sorted(mean):
-        print(f"  {k:<22} {mean[k]:.4g}  ({acc[k][1]} dispatches)")
+        print(f"  {k:<22} {mean[k]:.4g} per dispatch, {total[k]:.4g} in all")
     if "GRBM_GUI_ACTIVE" in mean:
         cyc = mean["GRBM_GUI_ACTIVE"] / xcds
         print(f"  cycles per XCD        {cyc:.4g}")
@@ -264,7 +279,6 @@ def sq_summary(paths: list, simds: int = 1024, xcds: int = 8) -> int:
         for w in ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY"):
             if w in mean:
                 print(f"  {w} / SQ_WAVE_CYCLES {mean[w] / mean['SQ_WAVE_CYCLES']:.3f}")
-    return 0
 
 
 def stats_summary(path: str) -> int:
@@ -450,6 +464,7 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--stats-summary", default=None)
     ap.add_argument("--sq-summary", nargs="+", default=None)
+    ap.add_argument("--sq-kernel", default=None, help="with --sq-summary: every kernel whose name contains this")
     ap.add_argument("--pmc-total", nargs=4, default=None)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -484,7 +499,7 @@ def main() -> int:
     if args.stats_summary:
         return stats_summary(args.stats_summary)
     if args.sq_summary:
-        return sq_summary(args.sq_summary)
+        return sq_summary(args.sq_summary, match=args.sq_kernel)
     if args.pmc_total:
         return pmc_total(args.pmc_total, args.copy_mib)
 
