@@ -173,22 +173,26 @@ int pick_band(int nx, int ny, int VW, int T, int C) {
   const int ntx = (nx + VW - 1) / VW;
   auto waves = [&](int H) { return static_cast<long long>(ntx) * ((ny + H - 1) / H); };
   if (C >= 2) {
-    static const int heights[] = {192, 160, 128, 112, 96, 80, 64, 48, 32};
+    // the tallest band that still gives about 6 waves per SIMD (16384^2: 192
+    // rows, 8192^2: 96); smaller grids take 64 (4096^2: 725 Gcell/s against
+    // 606 for one column per lane and 32-row bands)
+    static const int heights[] = {192, 160, 128, 112, 96, 80, 64};
     for (int H : heights)
       if (waves(H) >= 6LL * 1024) return std::max(H, T);
-    return std::max(32, T);
+    return std::max(64, T);
   }
   int H = 256;
   while (H > 32 && waves(H) < 8LL * 1024) H /= 2;
   return std::max(H, T);
 }
 
-// Columns per lane a plan starts with: two when the grid is large enough for
-// the wider tiles to give every SIMD about 4 waves at 64-row bands (8192^2:
-// 752 against 713 Gcell/s), else one.
+// Columns per lane a plan starts with: two when the grid gives every SIMD
+// about 2 such waves at 64-row bands (8192^2: 752 against 713 Gcell/s;
+// 4096^2: 725 against 606-616), else one (2048^2: one column per lane 324
+// against 295).
 int default_cols_per_lane(int nx, int ny) {
   const long long ntx2 = (nx + 111) / 112;  // 64*2 - 2*8 stored columns
-  return ntx2 * (ny / 64) >= 4096 ? 2 : 1;
+  return ntx2 * (ny / 64) >= 2048 ? 2 : 1;
 }
 
 
@@ -763,6 +767,9 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   pl->pooled = pooled;
   pl->T = T;
   pl->C = C;
+  // rows of prefetch: one with wider tiles (the fused launch at 8192^2: 838
+  // against 829 Gcell/s with two; 16384^2 835 against 827), two otherwise
+  pl->D = C >= 2 ? 1 : 2;
   pl->band = band;
   pl->band_cpml = band_cpml;
   pl->L = make_layout(p, events.size());

@@ -147,8 +147,18 @@ struct RowS {
   float hxy_b, hxy_c, ezy_b, ezy_c;
 };
 
+typedef int rec_v4i __attribute__((ext_vector_type(4)));
+typedef const rec_v4i __attribute__((address_space(4))) *rec_ptr;
+
+// Record of row r; levels() passes rec_base = the record of row k - T, so
+// that level t's record is a constant offset from it (one address per row
+// iteration instead of one per level).
+__device__ __forceinline__ rec_ptr row_record(const StepArgs &a, int r) {
+  return (rec_ptr)a.rowinfo + (kRowRec / 4) * r;
+}
+
 template <int K>
-__device__ __forceinline__ RowS row_scalars(const StepArgs &a, int r) {
+__device__ __forceinline__ RowS row_scalars(const StepArgs &a, rec_ptr rec) {
   RowS q;
   q.hxy_s = q.ezy_s = -1;
   q.hxy_b = q.hxy_c = q.ezy_b = q.ezy_c = 0.f;
@@ -166,16 +176,13 @@ __device__ __forceinline__ RowS row_scalars(const StepArgs &a, int r) {
   // single fields would be sunk into the branches that use them, each then
   // waiting for its own load); the records cover rows -kMaxT .. ny+kMaxT-1,
   // those outside the grid with no events and no strips
-  typedef int v4i __attribute__((ext_vector_type(4)));
-  typedef const v4i __attribute__((address_space(4))) *cv4i_ptr;
-  const cv4i_ptr rec = (cv4i_ptr)a.rowinfo + (kRowRec / 4) * r;
-  const v4i w0 = rec[0];
+  const rec_v4i w0 = rec[0];
   q.den_hy = __int_as_float(w0.x);
   q.den_ey = __int_as_float(w0.y);
   q.e0 = w0.z >= 0 ? w0.z : 0;
   q.e1 = w0.z >= 0 ? w0.w : 0;
   if constexpr (!(K & kY)) return q;  // event tiles: den and the event range only
-  const v4i w1 = rec[1], w2 = rec[2];
+  const rec_v4i w1 = rec[1], w2 = rec[2];
   q.hxy_s = w1.x;
   q.ezy_s = w1.y;
   q.hxy_b = __int_as_float(w1.z);
@@ -529,6 +536,7 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
   float nxt_ez[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) nxt_ez[c] = st[0].ez[c];
+  const rec_ptr rec_base = row_record(a, k - T);  // k - T >= -kMaxT: the records' lower padding
 #pragma unroll
   for (int t = 1; t <= T; ++t) {
     if constexpr (WARM) {
@@ -537,7 +545,7 @@ __device__ __forceinline__ void levels(const StepArgs &a, const float4 *lut_s, c
     const int r = k - t;
     Level<C, K> out;
     const bool own_row = r >= jb0 && r < jb1;
-    const RowS rs = row_scalars<K>(a, r);
+    const RowS rs = row_scalars<K>(a, rec_base + (kRowRec / 4) * (T - t));
     advance<C, K, MED, JZ, XT>(a, lut_s, cl, rs, prev, nxt_ez, st[t].hx, r, n0 + t, own0, own1, own_row, out);
     prev = st[t];
     st[t] = out;

@@ -133,11 +133,12 @@ class ShapeCache:
 
 
 # Plans: the launch shape of one grid, keyed by its size buckets, CPML
-# thickness and medium. Axes of the plan descent, first value the start.
+# thickness and medium. Axes of the plan descent (the plan's own shape is
+# the start).
 PLAN_AXES = [
-    ("cols_per_lane", [2, 1]),
-    ("band", [96, 64, 128, 192, 256]),
-    ("prefetch", [2, 1]),
+    ("cols_per_lane", [1, 2]),
+    ("band", [32, 64, 96, 128, 192, 256]),
+    ("prefetch", [1, 2]),
 ]
 
 
@@ -167,10 +168,7 @@ def tune_plan(plan, launches: int = 4, axes=PLAN_AXES, log=None) -> dict:
     real problem (results do not depend on the shape) and is scored by cell
     updates per second. One untimed trial first (cold caches). Stops when the
     problem runs out of steps. Leaves the plan at the best shape."""
-    start = {name: values[0] for name, values in axes}
-    best = dict(start)
-    for name, value in best.items():
-        plan.set_option(name, value)
+    best = {name: int(plan.info(name)) for name, _ in axes}  # the library's default shape first
     _plan_trial(plan, launches)
     best_rate = _plan_trial(plan, launches)
     trials = [(dict(best), best_rate)]

@@ -148,7 +148,7 @@ class FakePlan:
     counter and a step budget."""
 
     def __init__(self, nsteps=4000, refuse=()):
-        self.options = {"tblock": 8}
+        self.options = {"tblock": 8, "cols_per_lane": 2, "band": 96, "prefetch": 2}
         self.n = 0
         self.nsteps = nsteps
         self.nx = self.ny = 1000
@@ -162,7 +162,7 @@ class FakePlan:
         self.options[name] = value
 
     def info(self, name):
-        return {"tblock": 8, "steps": self.n}[name]
+        return {"steps": self.n, **self.options}[name]
 
     def run(self, steps):
         self.n += steps
