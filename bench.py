@@ -200,7 +200,7 @@ def pmc_total(paths: list, copy_mib: int) -> int:
                 name = row["Kernel_Name"]
                 if "fdtd2d::" not in name:
                     continue
-                short = name.split("(anonymous namespace)::")[-1].split("(")[0]
+                short = name.replace("(anonymous namespace)::", "").split("(")[0].split("::")[-1]
                 v = out.setdefault(short, [0.0, 0, 0])
                 v[0] += float(row["Counter_Value"])
                 v[1] += 1
