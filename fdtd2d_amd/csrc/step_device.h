@@ -583,10 +583,12 @@ __device__ __forceinline__ void row_step(const StepArgs &a, const Streams &dst, 
     const int pitch = a.pitch;
     const Level<C, K> &o = st[T];
     const long long rowp = static_cast<long long>(r) * pitch;
-    store_row<C>(a.ez[1 - src], rowp, cl.c0, own0, own1, o.ez);
-    store_row<C>(a.hx[1 - src], rowp, cl.c0, own0, own1, o.hx);
-    store_row<C>(a.hy[1 - src], rowp, cl.c0, own0, own1, o.hy);
-    store_row<C>(a.jz[1 - src], rowp, cl.c0, own0, own1, o.jz);
+    // the destination planes were resolved once per tile (dst), so the
+    // row loop issues no scalar loads for them
+    store_row<C>((float *)dst.ez, rowp, cl.c0, own0, own1, o.ez);
+    store_row<C>((float *)dst.hx, rowp, cl.c0, own0, own1, o.hx);
+    store_row<C>((float *)dst.hy, rowp, cl.c0, own0, own1, o.hy);
+    store_row<C>((float *)dst.jz, rowp, cl.c0, own0, own1, o.jz);
     if constexpr ((K & kY) != 0) {
       const int sh = cld(a.hxy_slot, r), se = cld(a.ezy_slot, r);
       if (sh >= 0) store_row<C>(a.psi_hxy[1 - src], static_cast<long long>(sh) * pitch, cl.c0, own0, own1, o.phxy);
