@@ -581,6 +581,8 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
   } else if (n == "fused") {
     if (v != 0 && v != 1) return set_error(FDTD2D_EINVAL, "fused must be 0 or 1");
     pp->fused = static_cast<int>(v);
+    // the CPML-class entries are routed to lists 1 and 2 by the arrangement
+    pp->lists_valid = false;
     pp->graph_valid = false;
   } else if (n == "wg_per_cu") {
     if (v < 1 || v > 8) return set_error(FDTD2D_EINVAL, "wg_per_cu must be 1..8");

@@ -103,6 +103,27 @@ def test_pool_reshape_midrun_and_late_admission(gpu_device):
             fp64_check(p, g["det"], f"reshaped pool shot {sid}", comp32=g["comp"])
 
 
+def test_pool_fused_toggled_midrun_table_medium(gpu_device):
+    """Switching between one kernel per tile class and the fused kernel (whose
+    table-medium form leaves corner and general pieces to a second kernel)
+    between launches re-routes the CPML-class entries; results stay equal to
+    the oracle."""
+    probs = _problems(5151, 5, table=True, nsteps=56)
+    with F.Pool(gpu_device, tblock=8, graph_chunk=2, fused=0) as pool:
+        ids = [pool.add(p) for p in probs]
+        pool.step(2)
+        pool.set_option("fused", 1)
+        pool.step(2)
+        pool.set_option("fused", 0)
+        pool.step(1)
+        pool.set_option("fused", 1)
+        _drain(pool, 2)
+        for sid, p in zip(ids, probs):
+            g = _pool_state(pool, sid)
+            _same(g, _oracle_state(p), f"shot {sid}")
+            fp64_check(p, g["det"], f"fused-toggled pool shot {sid}", comp32=g["comp"])
+
+
 def test_pool_budget_admission_and_slot_reuse(gpu_device):
     """Byte-accurate admission against an arena; removed shots free their
     bytes and slots for new ones, and results do not depend on neighbours."""
