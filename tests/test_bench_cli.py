@@ -78,3 +78,56 @@ def test_stats_summary_table(tmp_path, capsys):
     lines = capsys.readouterr().out.splitlines()
     assert lines[0].split()[:2] == ["calls", "avg_us"]
     assert lines[1].split()[:2] == ["52", "671.7"] and lines[1].endswith(INTERIOR)
+
+
+def _timed_csv(path: Path, rows):
+    """rows: (kernel, counter, value, start_ns, end_ns)"""
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp",
+                                          "End_Timestamp"])
+        w.writeheader()
+        for name, counter, value, t0, t1 in rows:
+            w.writerow({"Kernel_Name": name, "Counter_Name": counter, "Counter_Value": value, "Start_Timestamp": t0,
+                        "End_Timestamp": t1})
+
+
+def test_sq_summary_valu_issue_and_waits(tmp_path, capsys):
+    """VALU issue = instructions over one wave64 instruction per SIMD every 4
+    cycles in GRBM_GUI_ACTIVE / 8 cycles (the counter runs on every XCD)."""
+    a, b = tmp_path / "a.csv", tmp_path / "b.csv"
+    # 2 dispatches; per dispatch 8e6 GRBM cycles (1e6 per XCD) -> 1024 * 1e6 / 4 = 2.56e8 issue slots
+    _timed_csv(a, [(FUSED, "SQ_INSTS_VALU", 1.28e8, 0, 1000), (FUSED, "SQ_INSTS_VALU", 1.28e8, 2000, 3000),
+                   (FUSED, "SQ_WAVE_CYCLES", 1e9, 0, 1000), (FUSED, "SQ_WAVE_CYCLES", 1e9, 2000, 3000),
+                   (FUSED, "SQ_WAIT_INST_ANY", 2.5e8, 0, 1000), (FUSED, "SQ_WAIT_INST_ANY", 2.5e8, 2000, 3000),
+                   (COPY, "SQ_INSTS_VALU", 5.0, 0, 10)])
+    _timed_csv(b, [(FUSED, "GRBM_GUI_ACTIVE", 8e6, 0, 1000), (FUSED, "GRBM_GUI_ACTIVE", 8e6, 2000, 3000)])
+    assert bench.sq_summary([str(a), str(b)]) == 0
+    out = capsys.readouterr().out
+    assert "step_kernel<8, 2, 2, 1, 2>" in out and "copyBuffer" not in out
+    assert "VALU issue            0.500" in out
+    assert "SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES 0.250" in out
+    assert "dispatches 2" in out
+    # any kernel by name
+    assert bench.sq_summary([str(a)], match="copyBuffer") == 0
+    assert "copyBuffer" in capsys.readouterr().out
+
+
+def test_pmc_total_sums_every_dispatch(tmp_path, capsys):
+    """Calibrated bytes summed over every dispatch of the stepper's kernels,
+    over the summed dispatch time; other kernels are left out."""
+    pool0 = "void fdtd2d::(anonymous namespace)::pool_kernel<8, 1, 1, 1, 0>(fdtd2d::StepArgs const*, int)"
+    pset = "fdtd2d::(anonymous namespace)::pool_set(fdtd2d::PoolQueue*, fdtd2d::(anonymous namespace)::PoolLists, int)"
+    f, w, cf, cw = (tmp_path / n for n in ("f.csv", "w.csv", "cf.csv", "cw.csv"))
+    # 2 dispatches of 1 ms each: FETCH 1e6 KiB raw each (x2), WRITE 5e5 KiB each (x1)
+    _timed_csv(f, [(pool0, "FETCH_SIZE", 1e6, 0, 1_000_000), (pool0, "FETCH_SIZE", 1e6, 2_000_000, 3_000_000),
+                   (pset, "FETCH_SIZE", 0.0, 0, 100), (COPY, "FETCH_SIZE", 9e9, 0, 10)])
+    _timed_csv(w, [(pool0, "WRITE_SIZE", 5e5, 0, 1_000_000), (pool0, "WRITE_SIZE", 5e5, 2_000_000, 3_000_000),
+                   (pset, "WRITE_SIZE", 0.0, 0, 100)])
+    mib = 1024
+    _counter_csv(cf, "FETCH_SIZE", [(COPY, mib * 1024 / 2)])  # FETCH reads half: scale 2
+    _counter_csv(cw, "WRITE_SIZE", [(COPY, mib * 1024)])
+    assert bench.pmc_total([str(f), str(w), str(cf), str(cw)], mib) == 0
+    out = capsys.readouterr().out
+    assert "pool_kernel<8, 1, 1, 1, 0>" in out and "pool_set" in out and "copyBuffer" not in out
+    total = (2 * 1e6 * 1024 * 2 + 2 * 5e5 * 1024) / 1e9
+    assert f"total: {total:.2f} GB in 2.0 ms of dispatches" in out
