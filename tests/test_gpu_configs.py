@@ -149,26 +149,34 @@ def test_16384_shot_plan_and_pool(gpu_device):
         det, comp = _same_traces(plan, ref, "16384^2 plan")
     fp64_check(setup, det, "16384^2 16 steps", threads=THREADS, comp32=comp)
     smalls = [F.Setup.workload(5, seed=5, shot=s, steps=steps) for s in _small_shot_indices()]
-    with F.Pool(gpu_device, graph_chunk=2) as pool:
-        ids = [pool.add(setup)] + [pool.add(s) for s in smalls]
-        while pool.info("running") > 0:
-            pool.step(2)
-        pool.sync()
-        print(f"pool: launches {pool.info('launches'):.0f}, graph replays {pool.info('graph_replays'):.0f}",
-              flush=True)
-        assert pool.info("graph_replays") >= 1
-        shot = _PoolShot(pool, ids[0])
-        compare_fields_lean(shot, ref, "16384^2 in a pool", psi=False)
-        _same_traces(shot, ref, "16384^2 in a pool")
-        ref.close()
-        for sid, s in zip(ids[1:], smalls):
-            r = Oracle(s, 32, threads=THREADS)
-            r.run(steps)
-            small = _PoolShot(pool, sid)
-            compare_fields_lean(small, r, f"1024^2 shot {sid} beside it", psi=False)
-            d, c = _same_traces(small, r, f"1024^2 shot {sid}")
-            r.close()
-            fp64_check(s, d, f"1024^2 shot {sid} beside the 16384^2 shot", threads=THREADS, comp32=c)
+    small_refs = []
+    for s in smalls:
+        r = Oracle(s, 32, threads=THREADS)
+        r.run(steps)
+        small_refs.append(r)
+    # the pool's default shape, then the shape the serving stream of these
+    # shots takes from the shape cache (two columns per lane, 192-row bands)
+    for label, shape in (("default", {}), ("mixed-stream", {"cols_per_lane": 2, "band": 192, "prefetch": 2})):
+        with F.Pool(gpu_device, graph_chunk=2, **shape) as pool:
+            ids = [pool.add(setup)] + [pool.add(s) for s in smalls]
+            while pool.info("running") > 0:
+                pool.step(2)
+            pool.sync()
+            print(f"pool ({label} shape): launches {pool.info('launches'):.0f}, "
+                  f"graph replays {pool.info('graph_replays'):.0f}", flush=True)
+            assert pool.info("graph_replays") >= 1
+            shot = _PoolShot(pool, ids[0])
+            compare_fields_lean(shot, ref, f"16384^2 in a pool ({label} shape)", psi=False)
+            _same_traces(shot, ref, f"16384^2 in a pool ({label} shape)")
+            for sid, s, r in zip(ids[1:], smalls, small_refs):
+                small = _PoolShot(pool, sid)
+                compare_fields_lean(small, r, f"1024^2 shot {sid} beside it ({label} shape)", psi=False)
+                d, c = _same_traces(small, r, f"1024^2 shot {sid} ({label} shape)")
+                if label == "default":
+                    fp64_check(s, d, f"1024^2 shot {sid} beside the 16384^2 shot", threads=THREADS, comp32=c)
+    ref.close()
+    for r in small_refs:
+        r.close()
 
 
 def test_batch_mix_first_16_shots(gpu_device):
