@@ -78,3 +78,62 @@ def test_footprint_counts_both_buffer_sets():
 def test_oracle_library_loads():
     lib = O.lib()
     assert hasattr(lib, "oracle_create") and hasattr(lib, "oracle_run")
+
+
+C_CONSUMER = r"""
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "fdtd2d.h"
+int main(int argc, char **argv) {
+  if (argc < 2) return 2;
+  printf("version %d\n", fdtd2d_abi_version());
+  printf("problem %zu %zu %zu %zu %zu\n", sizeof(fdtd2d_problem), offsetof(fdtd2d_problem, nx),
+         offsetof(fdtd2d_problem, src_table), offsetof(fdtd2d_problem, det_i), offsetof(fdtd2d_problem, nrc));
+  printf("processing %zu\n", sizeof(fdtd2d_processing));
+  fdtd2d_setup *s = NULL;
+  int rc = fdtd2d_setup_from_ini(argv[1], &s);
+  if (rc) {
+    printf("error %d %s\n", rc, fdtd2d_last_error());
+    return 1;
+  }
+  const fdtd2d_problem *p = fdtd2d_setup_problem(s);
+  uint64_t bytes = 0;
+  rc = fdtd2d_plan_footprint(p, &bytes);
+  printf("grid %d %d %d %llu %d\n", p->nx, p->ny, p->nsteps, (unsigned long long)bytes, rc);
+  fdtd2d_setup_destroy(s);
+  return 0;
+}
+"""
+
+
+def test_header_is_plain_c_and_the_layouts_agree(tmp_path):
+    """A C99 caller (the kind a cgo / FFI binding compiles) builds against
+    include/fdtd2d.h with -pedantic -Werror, links the library, and sees the
+    struct layouts the Python mirror uses; host-only calls, no device."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    src = tmp_path / "consumer.c"
+    src.write_text(C_CONSUMER)
+    exe = tmp_path / "consumer"
+    libdir = L.LIB_PATH.parent
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", f"-I{HEADER.parent}", str(src),
+                    f"-L{libdir}", "-lfdtd2d", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    ini = Path(__file__).resolve().parent / "golden" / "settings_release.ini"
+    out = subprocess.run([str(exe), str(ini)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert out[0] == "version 4"
+    size, nx, src_table, det_i, nrc = (int(v) for v in out[1].split()[1:])
+    assert size == C.sizeof(L.ProblemC)
+    assert (nx, src_table, det_i, nrc) == (L.ProblemC.nx.offset, L.ProblemC.src_table.offset,
+                                           L.ProblemC.det_i.offset, L.ProblemC.nrc.offset)
+    assert int(out[2].split()[1]) == C.sizeof(L.ProcessingC)
+    gx, gy, steps, bytes_, rc = (int(v) for v in out[3].split()[1:])
+    s = F.Setup.from_ini(str(ini))
+    p = s.c_problem.contents
+    assert (gx, gy, steps, rc) == (p.nx, p.ny, p.nsteps, 0)
+    assert bytes_ == s.footprint()
