@@ -452,6 +452,9 @@ def bench_serve(args) -> int:
         "gcell_per_s": round(st["cell_updates"] / wall / 1e9, 2),
         "hbm_gbs_algorithmic": round(36 * st["cell_updates"] / wall / 1e9, 1),
         "wall_s": round(wall, 2),
+        "host_s": st["host_s"],
+        "pool_host_s": st["pool_host_s"],
+        "pool_counts": st["pool_counts"],
     }
     print(json.dumps(line), flush=True)
     return 0

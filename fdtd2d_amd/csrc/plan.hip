@@ -607,6 +607,65 @@ int upload(fdtd2d_plan *pl, size_t off, const void *src, size_t bytes) {
   return FDTD2D_OK;
 }
 
+// Appends the maximal runs [first, end) of cells with Ne > 0 in one row of n
+// cells; positive(i) says whether cell i has Ne > 0. Blocks of 64 cells with
+// no change of state are passed over after one counting pass (the plasma of
+// a row is one or a few long runs).
+template <class Positive>
+static void append_runs(int n, Positive positive, std::vector<int2> &runs) {
+  constexpr int kBlock = 64;
+  int run = -1;  // first column of the open run
+  for (int i = 0; i < n; i += kBlock) {
+    const int m = std::min(kBlock, n - i);
+    int count = 0;
+    for (int k = 0; k < m; ++k) count += positive(i + k);
+    if (count == (run >= 0 ? m : 0)) continue;
+    for (int k = 0; k < m; ++k) {
+      const bool pos = positive(i + k);
+      if (pos && run < 0) {
+        run = i + k;
+      } else if (!pos && run >= 0) {
+        runs.push_back(make_int2(run, i + k));
+        run = -1;
+      }
+    }
+  }
+  if (run >= 0) runs.push_back(make_int2(run, n));
+}
+
+// Per row of the problem: the runs of Ne > 0 (ne_runs, indexed by
+// ne_run_off) and their column span (ne_lo / ne_hi: first and last plasma
+// column, INT32_MAX / -1 for a row without plasma).
+static void plasma_runs(const fdtd2d_problem *p, fdtd2d_plan *pl) {
+  const int nx = p->nx, ny = p->ny;
+  pl->ne_lo.assign(ny, INT32_MAX);
+  pl->ne_hi.assign(ny, -1);
+  pl->ne_run_off.assign(1, 0);
+  pl->ne_runs.clear();
+  std::vector<uint8_t> mat_pos;
+  if (p->medium_kind == FDTD2D_MEDIUM_TABLE) {
+    mat_pos.resize(p->nmaterials);
+    for (int m = 0; m < p->nmaterials; ++m) mat_pos[m] = p->ne[m] > 0.f;
+  }
+  for (int j = 0; j < ny; ++j) {
+    const size_t row = static_cast<size_t>(j) * nx;
+    const size_t first = pl->ne_runs.size();
+    if (p->medium_kind == FDTD2D_MEDIUM_TABLE) {
+      const int32_t *mat = p->material + row;
+      const uint8_t *pos = mat_pos.data();
+      append_runs(nx, [=](int i) { return static_cast<int>(pos[mat[i]]); }, pl->ne_runs);
+    } else {
+      const float *ne = p->density + row;
+      append_runs(nx, [=](int i) { return static_cast<int>(ne[i] > 0.f); }, pl->ne_runs);
+    }
+    if (pl->ne_runs.size() > first) {
+      pl->ne_lo[j] = pl->ne_runs[first].x;
+      pl->ne_hi[j] = pl->ne_runs.back().y - 1;
+    }
+    pl->ne_run_off.push_back(static_cast<int>(pl->ne_runs.size()));
+  }
+}
+
 int drain_timing(fdtd2d_plan *pl) {
   for (int c = 0; c < 2; ++c) {
     for (auto &pr : pl->pending[c]) {
@@ -825,21 +884,28 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
   }
 
   const int nx = L.nx, ny = L.ny, pitch = L.pitch, P = L.P;
-  // medium in pitch layout
+  // the tables from the material table to the source table are staged in one
+  // host image and copied with one transfer (gaps between them are zero)
+  std::vector<char> img(L.off_det - L.off_lut, 0);
+  auto stage = [&](size_t off, const void *src, size_t bytes) {
+    if (bytes) std::memcpy(img.data() + (off - L.off_lut), src, bytes);
+    return FDTD2D_OK;
+  };
+  // medium in pitch layout: the material ids or the density bits, 4 bytes a
+  // cell either way (the pad columns stay zero)
   {
-    std::vector<uint32_t> med(static_cast<size_t>(ny) * pitch, 0u);
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) {
-        size_t c = static_cast<size_t>(j) * nx + i;
-        uint32_t v;
-        if (p->medium_kind == FDTD2D_MEDIUM_TABLE) {
-          v = static_cast<uint32_t>(p->material[c]);
-        } else {
-          std::memcpy(&v, &p->density[c], 4);
-        }
-        med[static_cast<size_t>(j) * pitch + i] = v;
-      }
-    if ((rc = upload(pl, L.off_med, med.data(), med.size() * 4))) return fail(rc);
+    const void *cells = p->medium_kind == FDTD2D_MEDIUM_TABLE ? static_cast<const void *>(p->material)
+                                                              : static_cast<const void *>(p->density);
+    if (pitch == nx) {
+      if ((rc = upload(pl, L.off_med, cells, static_cast<size_t>(ny) * nx * 4))) return fail(rc);
+    } else {
+      std::vector<uint32_t> med(static_cast<size_t>(ny) * pitch, 0u);
+      const char *src = static_cast<const char *>(cells);
+      for (int j = 0; j < ny; ++j)
+        std::memcpy(&med[static_cast<size_t>(j) * pitch], src + static_cast<size_t>(j) * nx * 4,
+                    static_cast<size_t>(nx) * 4);
+      if ((rc = upload(pl, L.off_med, med.data(), med.size() * 4))) return fail(rc);
+    }
   }
   if (p->medium_kind == FDTD2D_MEDIUM_TABLE) {
     std::vector<float> lut(static_cast<size_t>(p->nmaterials) * 4, 0.f);
@@ -848,7 +914,7 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       lut[4 * m + 1] = p->cb[m];
       lut[4 * m + 2] = p->ne[m];
     }
-    if ((rc = upload(pl, L.off_lut, lut.data(), lut.size() * 4))) return fail(rc);
+    if ((rc = stage(L.off_lut, lut.data(), lut.size() * 4))) return fail(rc);
   }
   // per-column tables: den_ex, den_hx, hyx_b, hyx_c, ezx_b, ezx_c, hyx_slot, ezx_slot
   {
@@ -880,9 +946,9 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       }
     }
     for (int f = 0; f < 6; ++f)
-      if ((rc = upload(pl, L.off_col[f], col[f].data(), pitch * 4))) return fail(rc);
-    if ((rc = upload(pl, L.off_col[6], hyx_s.data(), pitch * 4))) return fail(rc);
-    if ((rc = upload(pl, L.off_col[7], ezx_s.data(), pitch * 4))) return fail(rc);
+      if ((rc = stage(L.off_col[f], col[f].data(), pitch * 4))) return fail(rc);
+    if ((rc = stage(L.off_col[6], hyx_s.data(), pitch * 4))) return fail(rc);
+    if ((rc = stage(L.off_col[7], ezx_s.data(), pitch * 4))) return fail(rc);
   }
   // per-row tables: den_ey, den_hy, hxy_b, hxy_c, ezy_b, ezy_c, hxy_slot, ezy_slot
   {
@@ -914,9 +980,9 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       }
     }
     for (int f = 0; f < 6; ++f)
-      if ((rc = upload(pl, L.off_row[f], row[f].data(), ny * 4))) return fail(rc);
-    if ((rc = upload(pl, L.off_row[6], hxy_s.data(), ny * 4))) return fail(rc);
-    if ((rc = upload(pl, L.off_row[7], ezy_s.data(), ny * 4))) return fail(rc);
+      if ((rc = stage(L.off_row[f], row[f].data(), ny * 4))) return fail(rc);
+    if ((rc = stage(L.off_row[6], hxy_s.data(), ny * 4))) return fail(rc);
+    if ((rc = stage(L.off_row[7], ezy_s.data(), ny * 4))) return fail(rc);
     // the per-row record the CPML-class kernel reads at each level
     // (kMaxT records on each side for the rows the levels reach outside the
     // grid: no events, no strips)
@@ -947,7 +1013,7 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
       q[8] = bits(hrow ? row[4][j] : 0.f);
       q[9] = bits(hrow ? row[5][j] : 0.f);
     }
-    if ((rc = upload(pl, L.off_rowinfo, ri.data(), ri.size() * 4))) return fail(rc);
+    if ((rc = stage(L.off_rowinfo, ri.data(), ri.size() * 4))) return fail(rc);
   }
   // uniform interior den values (bitwise) and the rows / columns that differ
   float u_hx, u_ex, u_hy, u_ey;
@@ -985,31 +1051,11 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
               [](const int2 &u, const int2 &v) { return u.y != v.y ? u.y < v.y : u.x < v.x; });
   }
   // column spans of plasma per row; Jz starts at +0 everywhere
-  pl->ne_lo.assign(ny, INT32_MAX);
-  pl->ne_hi.assign(ny, -1);
+  plasma_runs(p, pl);
   pl->jz_lo.assign(ny, INT32_MAX);
   pl->jz_hi.assign(ny, -1);
-  pl->ne_run_off.assign(1, 0);
-  pl->ne_runs.clear();
-  for (int j = 0; j < ny; ++j) {
-    int run = -1;  // first column of the current run of Ne > 0
-    for (int i = 0; i < nx; ++i) {
-      const size_t c = static_cast<size_t>(j) * nx + i;
-      const float ne = p->medium_kind == FDTD2D_MEDIUM_TABLE ? p->ne[p->material[c]] : p->density[c];
-      if (ne > 0.f) {
-        pl->ne_lo[j] = std::min(pl->ne_lo[j], i);
-        pl->ne_hi[j] = std::max(pl->ne_hi[j], i);
-        if (run < 0) run = i;
-      } else if (run >= 0) {
-        pl->ne_runs.push_back(make_int2(run, i));
-        run = -1;
-      }
-    }
-    if (run >= 0) pl->ne_runs.push_back(make_int2(run, nx));
-    pl->ne_run_off.push_back(static_cast<int>(pl->ne_runs.size()));
-  }
-  if ((rc = upload(pl, L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
-  if ((rc = upload(pl, L.off_events, events.data(), events.size() * sizeof(fdtd2d::Event)))) return fail(rc);
+  if ((rc = stage(L.off_evoff, ev_off.data(), ev_off.size() * 4))) return fail(rc);
+  if ((rc = stage(L.off_events, events.data(), events.size() * sizeof(fdtd2d::Event)))) return fail(rc);
   // RC tracker: probe index per column, per-line probe pair, edge levels
   {
     std::vector<int32_t> pcol(pitch, -1);
@@ -1031,8 +1077,9 @@ int fdtd2d::detail::create_plan(const fdtd2d_problem *p, int device, bool pooled
     if ((rc = upload(pl, L.off_rcmap, map.data(), map.size() * 4))) return fail(rc);
     if ((rc = upload(pl, L.off_rcedge, edge.data(), edge.size() * 4))) return fail(rc);
   }
-  if ((rc = upload(pl, L.off_src, p->src_table, static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4)))
+  if ((rc = stage(L.off_src, p->src_table, static_cast<size_t>(p->nsteps + 1) * p->nsrc * 4)))
     return fail(rc);
+  if ((rc = upload(pl, L.off_lut, img.data(), img.size()))) return fail(rc);
 
   fdtd2d::StepArgs &a = pl->a;
   a.nx = nx;

@@ -172,11 +172,20 @@ struct fdtd2d_pool {
   // stats
   double cell_updates = 0;
   int graph_replays = 0, eager_launches = 0, tail_launches = 0;
+  int rebuilds = 0, captures = 0;
+  // host seconds: rebuilding and uploading main lists, capturing graphs,
+  // blocked on launches still running (outside fdtd2d_pool_wait/_sync),
+  // and adding shots
+  double rebuild_s = 0, capture_s = 0, blocked_s = 0, add_s = 0;
 };
 
 namespace {
 
 void retire(fdtd2d_pool *pp, int wait_g);
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 
 // Appends shot s's tile entries to the pool lists, tagged with its slot, each
 // list padded to whole work-groups (one shot per work-group). A fused pool
@@ -215,11 +224,14 @@ int upload_lists(const std::vector<int4> *lists, int4 **d, size_t *cap, int coun
 }
 
 int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
+  const auto t0 = std::chrono::steady_clock::now();
   std::vector<int4> lists[fdtd2d::kPoolLists];
   for (int s : slots) append_shot(pp, s, lists);
   const int nb = 1 - pp->cur;
   fdtd2d_pool::ListBuf &b = pp->bufs[nb];
+  const auto tw = std::chrono::steady_clock::now();
   retire(pp, b.last_g + 1);  // launches that read this buffer have finished
+  pp->blocked_s += seconds_since(tw);
   int count[fdtd2d::kPoolLists];
   int rc = upload_lists(lists, b.d, b.cap, count);
   if (rc) return rc;
@@ -232,6 +244,8 @@ int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
   pp->cur = nb;
   pp->main_set = slots;
   pp->lists_valid = true;
+  pp->rebuilds++;
+  pp->rebuild_s += seconds_since(t0);
   return FDTD2D_OK;
 }
 
@@ -320,6 +334,8 @@ int launch_main(fdtd2d_pool *pp, int k) {
 }
 
 int capture_graph(fdtd2d_pool *pp) {
+  const auto t0 = std::chrono::steady_clock::now();
+  pp->captures++;
   if (pp->gexec) {
     hipGraphExecDestroy(pp->gexec);
     pp->gexec = nullptr;
@@ -342,6 +358,7 @@ int capture_graph(fdtd2d_pool *pp) {
   hipGraphDestroy(graph);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   pp->graph_valid = true;
+  pp->capture_s += seconds_since(t0);
   return FDTD2D_OK;
 }
 
@@ -498,12 +515,17 @@ Shot *get_shot(fdtd2d_pool *pp, int id) {
 // Waits until the launch that issued the shot's last steps (or, for a shot
 // still running, everything issued) has finished on the device.
 int wait_shot(fdtd2d_pool *pp, const Shot &sh) {
+  const auto t0 = std::chrono::steady_clock::now();
   if (sh.finished) {
     retire(pp, sh.last_g + 1);
-    if (pp->g_done_known > sh.last_g) return FDTD2D_OK;
+    if (pp->g_done_known > sh.last_g) {
+      pp->blocked_s += seconds_since(t0);
+      return FDTD2D_OK;
+    }
   }
   HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
   retire(pp, -1);
+  pp->blocked_s += seconds_since(t0);
   return FDTD2D_OK;
 }
 
@@ -633,6 +655,12 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value)
   else if (n == "graph_replays") *value = pp->graph_replays;
   else if (n == "eager_launches") *value = pp->eager_launches;
   else if (n == "tail_launches") *value = pp->tail_launches;
+  else if (n == "rebuilds") *value = pp->rebuilds;
+  else if (n == "captures") *value = pp->captures;
+  else if (n == "rebuild_s") *value = pp->rebuild_s;
+  else if (n == "capture_s") *value = pp->capture_s;
+  else if (n == "blocked_s") *value = pp->blocked_s;
+  else if (n == "add_s") *value = pp->add_s;
   else if (n == "cus") *value = pp->ncu;
   else return set_error(FDTD2D_EINVAL, "unknown pool info " + n);
   return FDTD2D_OK;
@@ -647,6 +675,7 @@ int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
   if (pp->medium_kind >= 0 && p->medium_kind != pp->medium_kind)
     return set_error(FDTD2D_EUNSUPPORTED, "a pool holds shots of one medium kind");
   HIP_TRY(hipSetDevice(pp->device), "hipSetDevice");
+  const auto t0 = std::chrono::steady_clock::now();
   const size_t bytes = plan_bytes(p);
   char *base = nullptr;
   if (pp->budget > 0) {
@@ -700,6 +729,7 @@ int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
   pp->medium_kind = p->medium_kind;
   if ((rc = upload_desc(pp, slot))) return rc;
   *shot = slot;
+  pp->add_s += seconds_since(t0);
   return FDTD2D_OK;
 }
 

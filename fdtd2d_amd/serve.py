@@ -121,9 +121,11 @@ class Server:
         cells0 = pool.info("cell_updates")
         n_shots = 0
         last_log = 0.0
+        host = {"admit": 0.0, "step": 0.0, "wait": 0.0, "poll": 0.0, "complete": 0.0}
+        clock = time.perf_counter
         try:
             while True:
-                now = time.perf_counter() - t0
+                now = clock() - t0
                 # arrivals -> problem builds
                 while pending and pending[0].arrival <= now:
                     rq = pending.popleft()
@@ -138,6 +140,7 @@ class Server:
                     ready.append((s, s.footprint(), rq, pos))
                 # admission: FIFO, with backfill around a head that does not fit
                 if ready:
+                    ta = clock()
                     free = pool.info("largest_free")
                     keep = deque()
                     blocked = False
@@ -152,15 +155,21 @@ class Server:
                         running[sid] = (rq, pos)
                         free = pool.info("largest_free")
                     ready = keep
+                    host["admit"] += clock() - ta
                 # device work: keep at most max_in_flight launches queued ahead
                 if running:
+                    ta = clock()
                     pool.step(self.chunk)
+                    tb = clock()
                     pool.wait(self.max_in_flight)
+                    host["step"] += tb - ta
+                    host["wait"] += clock() - tb
                 # completions
-                for sid in list(running):
-                    _, _, complete = pool.state(sid)
-                    if not complete:
-                        continue
+                ta = clock()
+                finished = [sid for sid in running if pool.state(sid)[2]]
+                tb = clock()
+                host["poll"] += tb - ta
+                for sid in finished:
                     rq, pos = running.pop(sid)
                     det, _ = pool.read_detectors(sid)
                     pool.remove(sid)
@@ -169,8 +178,9 @@ class Server:
                         rq.results[pos] = det
                     rq.done += 1
                     if rq.done == len(rq.shots):
-                        rq.finished = time.perf_counter() - t0
+                        rq.finished = clock() - t0
                         done.append(rq)
+                host["complete"] += clock() - tb
                 if not pending and not to_build and not building and not ready and not running:
                     break
                 if now > max_seconds:
@@ -185,9 +195,13 @@ class Server:
             pool.sync()
         finally:
             ex.shutdown(wait=True, cancel_futures=True)
-        wall = time.perf_counter() - t0
+        wall = clock() - t0
         return done, {"wall": wall, "shots": n_shots, "cell_updates": pool.info("cell_updates") - cells0,
-                      "unfinished_requests": len(requests) - len(done)}
+                      "unfinished_requests": len(requests) - len(done),
+                      "host_s": {k: round(v, 3) for k, v in host.items()},
+                      "pool_host_s": {k: round(pool.info(k), 3) for k in ("add_s", "rebuild_s", "capture_s", "blocked_s")},
+                      "pool_counts": {k: int(pool.info(k)) for k in ("launches", "graph_replays", "eager_launches",
+                                                                      "tail_launches", "rebuilds", "captures")}}
 
 
 def latency_summary(done, classes=None):

@@ -422,7 +422,11 @@ int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *cou
  * and refuse shots that do not fit, FDTD2D_ENOMEM).
  * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",
  * "shots", "running", "launches", "cell_updates", "graph_replays",
- * "eager_launches", "tail_launches", "cus". */
+ * "eager_launches", "tail_launches", "cus", and host-side counters:
+ * "rebuilds" / "captures" (main tile lists rebuilt, graphs captured),
+ * "rebuild_s", "capture_s", "add_s" (host seconds spent in them and in
+ * fdtd2d_pool_add), "blocked_s" (host seconds waiting for launches still on
+ * the device, outside fdtd2d_pool_wait and fdtd2d_pool_sync). */
 typedef struct fdtd2d_pool fdtd2d_pool;
 int fdtd2d_pool_create(int device, fdtd2d_pool **out);
 int fdtd2d_pool_set_option(fdtd2d_pool *pool, const char *name, double value);
