@@ -144,15 +144,18 @@ struct fdtd2d_pool {
   fdtd2d::StepArgs *d_descs = nullptr;
   std::vector<fdtd2d::StepArgs> h_descs;  // host copy of the descriptor table
   size_t cap_slots = 0;
-  // main lists, double-buffered: a rebuild writes the buffer the launches in
-  // flight do not read, once the last launch that read it has finished
+  // main lists in a ring of `nbufs` buffers: a rebuild writes the buffer
+  // after the current one once the last launch that read it has finished, so
+  // the launches of the nbufs - 1 list versions before keep running meanwhile
   // (lists: 0 interior, 1 CPML class or its fusable kinds, 2 the leftover
   // CPML-class entries of a fused pool)
+  static constexpr int kListBufs = 4;
   struct ListBuf {
     int4 *d[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
     size_t cap[fdtd2d::kPoolLists] = {0, 0, 0};
     int last_g = -1;  // last pool launch that read it
-  } bufs[2];
+  } bufs[kListBufs];
+  int nbufs = kListBufs;
   int cur = 0;
   int main_count[fdtd2d::kPoolLists] = {0, 0, 0};  // work-groups per list of the current main lists
   int4 *d_tail[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
@@ -227,7 +230,7 @@ int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<int4> lists[fdtd2d::kPoolLists];
   for (int s : slots) append_shot(pp, s, lists);
-  const int nb = 1 - pp->cur;
+  const int nb = (pp->cur + 1) % pp->nbufs;
   fdtd2d_pool::ListBuf &b = pp->bufs[nb];
   const auto tw = std::chrono::steady_clock::now();
   retire(pp, b.last_g + 1);  // launches that read this buffer have finished
@@ -618,6 +621,9 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
     if (v < 0 || v > fdtd2d::kMaxChunk) return set_error(FDTD2D_EINVAL, "graph_chunk must be 0..16");
     pp->chunk = static_cast<int>(v);
     pp->graph_valid = false;
+  } else if (n == "list_buffers") {
+    if (v < 2 || v > fdtd2d_pool::kListBufs) return set_error(FDTD2D_EINVAL, "list_buffers must be 2..4");
+    pp->nbufs = static_cast<int>(v);  // from the next rebuild on; each buffer keeps its last reader
   } else {
     return set_error(FDTD2D_EINVAL, "unknown pool option " + n);
   }
@@ -645,6 +651,7 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value)
   else if (n == "fused") *value = pp->fused;
   else if (n == "wg_per_cu_cpml") *value = pp->wg_per_cu_cpml;
   else if (n == "graph_chunk") *value = pp->chunk;
+  else if (n == "list_buffers") *value = pp->nbufs;
   else if (n == "budget_bytes") *value = static_cast<double>(pp->budget);
   else if (n == "bytes_in_use") *value = static_cast<double>(pp->arena.in_use);
   else if (n == "largest_free") *value = static_cast<double>(pp->budget ? pp->arena.largest_free() : 0);
@@ -830,7 +837,7 @@ int fdtd2d_pool_destroy(fdtd2d_pool *pp) {
   for (auto e : pp->spare_events) hipEventDestroy(e);
   if (pp->gexec) hipGraphExecDestroy(pp->gexec);
   for (int c = 0; c < fdtd2d::kPoolLists; ++c) {
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < fdtd2d_pool::kListBufs; ++b)
       if (pp->bufs[b].d[c]) hipFree(pp->bufs[b].d[c]);
     if (pp->d_tail[c]) hipFree(pp->d_tail[c]);
   }

@@ -417,7 +417,9 @@ int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *cou
  * CPML-class kinds it does not carry in a second kernel beside it), "wg_per_cu" /
  * "wg_per_cu_cpml" (persistent work-groups per CU of the interior or fused
  * kernel / of the CPML-class kernel, default 3 / 2), "graph_chunk" (0..16,
- * 0 = no graphs),
+ * 0 = no graphs), "list_buffers" (2..4, default 4: device copies of the
+ * tile lists; a rebuild of the lists waits for the launches that read the
+ * copy it reuses),
  * "budget_bytes" (before any shot: carve shots from one arena of that size
  * and refuse shots that do not fit, FDTD2D_ENOMEM).
  * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",
