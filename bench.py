@@ -426,6 +426,12 @@ def bench_serve(args) -> int:
     for name, value in shape.items():
         srv.pool.set_option(name, value)
     log(f"pool shape {shape} ({source}; stream mix {key})")
+    # time slicing: the 16384^2 shots of the mixed stream sit out launches so
+    # that the 1024^2 shots do not wait for a full large launch every step
+    every = args.slice_every if args.slice_every >= 0 else (8 if args.mode == "serve-mixed" else 1)
+    slicing = {"slice_every": every, "slice_cells": args.slice_cells if every > 1 else 0}
+    for name, value in slicing.items():
+        srv.pool.set_option(name, value)
     done, st = srv.run(reqs, max_seconds=args.duration + args.drain, log=log)
     srv.close()
     lat = latency_summary(done, classes)
@@ -445,6 +451,7 @@ def bench_serve(args) -> int:
         "pool_shape": shape,
         "shape_source": source,
         "mix_key": key,
+        "time_slicing": slicing,
         "latency_s": lat,
         "completed_requests": len(done),
         "unfinished_requests": st["unfinished_requests"],
@@ -481,6 +488,10 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=0.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--drain", type=float, default=120.0)
+    ap.add_argument("--slice-every", type=int, default=-1,
+                    help="serving: shots of at least --slice-cells cells run in every Nth launch only "
+                         "(1 = off; default 8 for serve-mixed, off for serve-sweep)")
+    ap.add_argument("--slice-cells", type=float, default=16e6)
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--plan-shape", choices=["cache", "tune", "defaults"], default="cache",

@@ -96,9 +96,10 @@ struct StepArgs {
   const int4 *tile_list[2];
   int n_tiles[2];  // entries in use
   int n_wg[2];     // work-groups launched (entries / 4, rounded up)
-  // pool mode: at pool launch g this shot runs steps n_base + (g - g_base)*tp
-  // onwards from buffer set src_base ^ ((g - g_base) & 1)
-  int g_base, n_base, src_base;
+  // pool mode: the shot runs in launches g_base, g_base + g_stride, ...; at
+  // its d-th such launch (d = (g - g_base) / g_stride) it runs steps
+  // n_base + d*tp onwards from buffer set src_base ^ (d & 1)
+  int g_base, n_base, src_base, g_stride;
 };
 
 // Work queue of the pool kernels, in device memory. items/count: the packed

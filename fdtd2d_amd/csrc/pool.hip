@@ -144,22 +144,35 @@ struct fdtd2d_pool {
   fdtd2d::StepArgs *d_descs = nullptr;
   std::vector<fdtd2d::StepArgs> h_descs;  // host copy of the descriptor table
   size_t cap_slots = 0;
-  // main lists in a ring of `nbufs` buffers: a rebuild writes the buffer
-  // after the current one once the last launch that read it has finished, so
-  // the launches of the nbufs - 1 list versions before keep running meanwhile
-  // (lists: 0 interior, 1 CPML class or its fusable kinds, 2 the leftover
-  // CPML-class entries of a fused pool)
+  // main lists in a ring of `nbufs` buffers, each remembering the shot set
+  // it holds: a launch whose set one of them holds switches to it (two sets
+  // alternate under time slicing); otherwise the lists are rebuilt into the
+  // least recently read buffer once the last launch that read it has
+  // finished, so the launches of the other list versions keep running
+  // meanwhile (lists: 0 interior, 1 CPML class or its fusable kinds, 2 the
+  // leftover CPML-class entries of a fused pool)
   static constexpr int kListBufs = 4;
   struct ListBuf {
     int4 *d[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
     size_t cap[fdtd2d::kPoolLists] = {0, 0, 0};
     int last_g = -1;  // last pool launch that read it
+    bool valid = false;
+    std::vector<int> set;  // slots whose tiles it holds
+    int count[fdtd2d::kPoolLists] = {0, 0, 0};
   } bufs[kListBufs];
   int nbufs = kListBufs;
   int cur = 0;
   int main_count[fdtd2d::kPoolLists] = {0, 0, 0};  // work-groups per list of the current main lists
-  int4 *d_tail[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
-  size_t cap_tail[fdtd2d::kPoolLists] = {0, 0, 0};
+  // tail lists: one copy per tail group, taken in turn; a launch has at most
+  // T - 1 <= 7 groups, so a copy is reused only after launches whose
+  // completion events exist, and is waited for through them
+  static constexpr int kTailBufs = 8;
+  struct TailBuf {
+    int4 *d[fdtd2d::kPoolLists] = {nullptr, nullptr, nullptr};
+    size_t cap[fdtd2d::kPoolLists] = {0, 0, 0};
+    int last_g = -1;
+  } tailbufs[kTailBufs];
+  int tail_next = 0;
   fdtd2d::PoolQueue *d_q = nullptr, *d_qt = nullptr;
   // host bookkeeping
   int g = 0;                 // pool launches issued
@@ -175,7 +188,11 @@ struct fdtd2d_pool {
   // stats
   double cell_updates = 0;
   int graph_replays = 0, eager_launches = 0, tail_launches = 0;
-  int rebuilds = 0, captures = 0;
+  int rebuilds = 0, captures = 0, list_switches = 0;
+  // time slicing: shots of at least slice_cells cells run only in launches g
+  // with g % slice_every == 0 (0 cells: off)
+  double slice_cells = 0;
+  int slice_every = 1;
   // host seconds: rebuilding and uploading main lists, capturing graphs,
   // blocked on launches still running (outside fdtd2d_pool_wait/_sync),
   // and adding shots
@@ -226,30 +243,62 @@ int upload_lists(const std::vector<int4> *lists, int4 **d, size_t *cap, int coun
   return FDTD2D_OK;
 }
 
+// Makes ring buffer `nb` (holding the lists of `slots`) the current lists.
+int use_lists(fdtd2d_pool *pp, int nb, const std::vector<int> &slots) {
+  const fdtd2d_pool::ListBuf &b = pp->bufs[nb];
+  hipError_t e = fdtd2d::launch_pool_set(pp->d_q, b.d, b.count, -1, pp->stream);
+  if (e != hipSuccess) return hip_fail(e, "pool set launch");
+  // a captured graph holds the leftover kernel's grid (none when list 2 is empty)
+  const int gc = pp->ncu * pp->wg_per_cu_cpml;
+  if (std::min(gc, b.count[2]) != std::min(gc, pp->main_count[2])) pp->graph_valid = false;
+  for (int c = 0; c < fdtd2d::kPoolLists; ++c) pp->main_count[c] = b.count[c];
+  pp->cur = nb;
+  pp->main_set = slots;
+  pp->lists_valid = true;
+  return FDTD2D_OK;
+}
+
 int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
+  for (int i = 0; i < pp->nbufs; ++i)
+    if (pp->bufs[i].valid && pp->bufs[i].set == slots) {
+      pp->list_switches++;
+      return use_lists(pp, i, slots);
+    }
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<int4> lists[fdtd2d::kPoolLists];
   for (int s : slots) append_shot(pp, s, lists);
-  const int nb = (pp->cur + 1) % pp->nbufs;
+  // an unused buffer, else the least recently read one other than the current
+  int nb = -1;
+  for (int i = 0; i < pp->nbufs; ++i) {
+    if (i == pp->cur && pp->nbufs > 1) continue;
+    if (nb < 0 || (!pp->bufs[i].valid && pp->bufs[nb].valid) ||
+        (pp->bufs[i].valid == pp->bufs[nb].valid && pp->bufs[i].last_g < pp->bufs[nb].last_g))
+      nb = i;
+  }
   fdtd2d_pool::ListBuf &b = pp->bufs[nb];
   const auto tw = std::chrono::steady_clock::now();
   retire(pp, b.last_g + 1);  // launches that read this buffer have finished
   pp->blocked_s += seconds_since(tw);
-  int count[fdtd2d::kPoolLists];
-  int rc = upload_lists(lists, b.d, b.cap, count);
+  b.valid = false;
+  int rc = upload_lists(lists, b.d, b.cap, b.count);
   if (rc) return rc;
-  hipError_t e = fdtd2d::launch_pool_set(pp->d_q, b.d, count, -1, pp->stream);
-  if (e != hipSuccess) return hip_fail(e, "pool set launch");
-  // a captured graph holds the leftover kernel's grid (none when list 2 is empty)
-  const int gc = pp->ncu * pp->wg_per_cu_cpml;
-  if (std::min(gc, count[2]) != std::min(gc, pp->main_count[2])) pp->graph_valid = false;
-  for (int c = 0; c < fdtd2d::kPoolLists; ++c) pp->main_count[c] = count[c];
-  pp->cur = nb;
-  pp->main_set = slots;
-  pp->lists_valid = true;
+  b.set = slots;
+  b.valid = true;
+  if ((rc = use_lists(pp, nb, slots))) return rc;
   pp->rebuilds++;
   pp->rebuild_s += seconds_since(t0);
   return FDTD2D_OK;
+}
+
+// Forgets every list copy (tilings or routing changed).
+void invalidate_lists(fdtd2d_pool *pp) {
+  for (auto &b : pp->bufs) b.valid = false;
+  pp->lists_valid = false;
+}
+
+bool sliced(const fdtd2d_pool *pp, const fdtd2d_plan *pl) {
+  return pp->slice_every > 1 && pp->slice_cells > 0 &&
+         static_cast<double>(pl->L.nx) * pl->L.ny >= pp->slice_cells;
 }
 
 // Fills the host copy of shot `slot`'s descriptor: its state at the next
@@ -258,7 +307,10 @@ void fill_desc(fdtd2d_pool *pp, int slot) {
   const fdtd2d_plan *pl = pp->shots[slot].pl;
   fdtd2d::StepArgs &a = pp->h_descs[slot];
   a = pl->a;
-  a.g_base = pp->g;
+  // a sliced shot runs in every slice_every-th launch from the next one on
+  const int k = sliced(pp, pl) ? pp->slice_every : 1;
+  a.g_stride = k;
+  a.g_base = (pp->g + k - 1) / k * k;
   a.n_base = pl->n;
   a.src_base = pl->src;
 }
@@ -374,14 +426,18 @@ int run_tails(fdtd2d_pool *pp, const std::vector<int> &tails) {
     const int t = kv.first;
     std::vector<int4> lists[fdtd2d::kPoolLists];
     for (int s : kv.second) append_shot(pp, s, lists);
-    // the tail buffers are reused: wait until the previous tail launch is done
-    HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+    fdtd2d_pool::TailBuf &tb = pp->tailbufs[pp->tail_next];
+    pp->tail_next = (pp->tail_next + 1) % fdtd2d_pool::kTailBufs;
+    const auto tw = std::chrono::steady_clock::now();
+    retire(pp, tb.last_g + 1);  // the launch that last read this copy has finished
+    pp->blocked_s += seconds_since(tw);
+    tb.last_g = pp->g;
     int count[fdtd2d::kPoolLists];
-    int rc = upload_lists(lists, pp->d_tail, pp->cap_tail, count);
+    int rc = upload_lists(lists, tb.d, tb.cap, count);
     if (rc) return rc;
     const int n0 = count[0], n1 = count[1], n2 = count[2];
     const int gi = pp->ncu * pp->wg_per_cu, gc = pp->ncu * pp->wg_per_cu_cpml;
-    hipError_t e = fdtd2d::launch_pool_set(pp->d_qt, pp->d_tail, count, pp->g, pp->stream);
+    hipError_t e = fdtd2d::launch_pool_set(pp->d_qt, tb.d, count, pp->g, pp->stream);
     if (pp->fused) {
       // in stream order: the leftover kernel's tiles are disjoint from the
       // fused kernel's, both read set src and write set 1 - src
@@ -430,7 +486,7 @@ int reshape(fdtd2d_pool *pp) {
   }
   int rc = copy_descs(pp, 0, pp->shots.size());
   if (rc) return rc;
-  pp->lists_valid = false;
+  invalidate_lists(pp);
   pp->graph_valid = false;
   return FDTD2D_OK;
 }
@@ -440,9 +496,16 @@ int step_impl(fdtd2d_pool *pp, int nlaunch) {
   while (nlaunch > 0) {
     std::vector<int> main_set, tails;
     int min_left = 1 << 30;
+    // time slicing: sliced shots sit out the launches between their turns
+    const bool slice_turn = pp->g % std::max(pp->slice_every, 1) == 0;
+    bool any_sliced = false;
     for (size_t s = 0; s < pp->shots.size(); ++s) {
       const Shot &sh = pp->shots[s];
       if (!sh.live || sh.finished) continue;
+      if (sliced(pp, sh.pl)) {
+        any_sliced = true;
+        if (!slice_turn) continue;
+      }
       const int left = sh.pl->nsteps - sh.pl->n;
       if (left >= pp->T) {
         main_set.push_back(static_cast<int>(s));
@@ -451,13 +514,22 @@ int step_impl(fdtd2d_pool *pp, int nlaunch) {
         tails.push_back(static_cast<int>(s));
       }
     }
-    if (main_set.empty() && tails.empty()) break;
+    if (main_set.empty() && tails.empty()) {
+      if (!any_sliced) break;
+      // only sliced shots run and this is not their turn: move the launch
+      // counter (host and device) on to their next turn, launching nothing
+      const int skip = pp->slice_every - pp->g % pp->slice_every;
+      hipError_t e = fdtd2d::launch_pool_tick(pp->d_q, 0, skip, pp->stream);
+      if (e != hipSuccess) return hip_fail(e, "pool tick launch");
+      pp->g += skip;
+      continue;
+    }
     if (!pp->lists_valid || main_set != pp->main_set) {
       int rc = rebuild_main(pp, main_set);
       if (rc) return rc;
     }
     const bool use_graph = pp->chunk > 0 && nlaunch >= pp->chunk && tails.empty() && !main_set.empty() &&
-                           min_left >= pp->chunk * pp->T;
+                           min_left >= pp->chunk * pp->T && !any_sliced;
     if (use_graph) {
       if (!pp->graph_valid) {
         int rc = capture_graph(pp);
@@ -607,7 +679,7 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
     if (v != 0 && v != 1) return set_error(FDTD2D_EINVAL, "fused must be 0 or 1");
     pp->fused = static_cast<int>(v);
     // the CPML-class entries are routed to lists 1 and 2 by the arrangement
-    pp->lists_valid = false;
+    invalidate_lists(pp);
     pp->graph_valid = false;
   } else if (n == "wg_per_cu") {
     if (v < 1 || v > 8) return set_error(FDTD2D_EINVAL, "wg_per_cu must be 1..8");
@@ -624,6 +696,17 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
   } else if (n == "list_buffers") {
     if (v < 2 || v > fdtd2d_pool::kListBufs) return set_error(FDTD2D_EINVAL, "list_buffers must be 2..4");
     pp->nbufs = static_cast<int>(v);  // from the next rebuild on; each buffer keeps its last reader
+  } else if (n == "slice_cells" || n == "slice_every") {
+    if (n == "slice_cells" && !(v >= 0)) return set_error(FDTD2D_EINVAL, "slice_cells must be >= 0");
+    if (n == "slice_every" && (v < 1 || v > 64)) return set_error(FDTD2D_EINVAL, "slice_every must be 1..64");
+    // the shots' descriptors count their turns from the next launch on
+    HIP_TRY(hipStreamSynchronize(pp->stream), "hipStreamSynchronize");
+    retire(pp, pp->g);
+    if (n == "slice_cells") pp->slice_cells = v;
+    else pp->slice_every = static_cast<int>(v);
+    for (size_t s = 0; s < pp->shots.size(); ++s)
+      if (pp->shots[s].live) fill_desc(pp, static_cast<int>(s));
+    return copy_descs(pp, 0, pp->shots.size());
   } else {
     return set_error(FDTD2D_EINVAL, "unknown pool option " + n);
   }
@@ -652,6 +735,9 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value)
   else if (n == "wg_per_cu_cpml") *value = pp->wg_per_cu_cpml;
   else if (n == "graph_chunk") *value = pp->chunk;
   else if (n == "list_buffers") *value = pp->nbufs;
+  else if (n == "slice_cells") *value = pp->slice_cells;
+  else if (n == "slice_every") *value = pp->slice_every;
+  else if (n == "list_switches") *value = pp->list_switches;
   else if (n == "budget_bytes") *value = static_cast<double>(pp->budget);
   else if (n == "bytes_in_use") *value = static_cast<double>(pp->arena.in_use);
   else if (n == "largest_free") *value = static_cast<double>(pp->budget ? pp->arena.largest_free() : 0);
@@ -820,6 +906,9 @@ int fdtd2d_pool_remove(fdtd2d_pool *pp, int id) {
   pp->free_slots.push_back(id);
   auto it = std::find(pp->main_set.begin(), pp->main_set.end(), id);
   if (it != pp->main_set.end()) pp->lists_valid = false;
+  // the slot may come back with another shot: no list copy may name it
+  for (auto &b : pp->bufs)
+    if (b.valid && std::find(b.set.begin(), b.set.end(), id) != b.set.end()) b.valid = false;
   return FDTD2D_OK;
 }
 
@@ -839,7 +928,8 @@ int fdtd2d_pool_destroy(fdtd2d_pool *pp) {
   for (int c = 0; c < fdtd2d::kPoolLists; ++c) {
     for (int b = 0; b < fdtd2d_pool::kListBufs; ++b)
       if (pp->bufs[b].d[c]) hipFree(pp->bufs[b].d[c]);
-    if (pp->d_tail[c]) hipFree(pp->d_tail[c]);
+    for (auto &tb : pp->tailbufs)
+      if (tb.d[c]) hipFree(tb.d[c]);
   }
   if (pp->d_descs) hipFree(pp->d_descs);
   if (pp->d_q) hipFree(pp->d_q);

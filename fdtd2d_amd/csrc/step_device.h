@@ -920,7 +920,7 @@ __global__ void __launch_bounds__(64 * kWavesPerWG, MODE == 0 ? (C > 1 && D > 1 
       __syncthreads();
     }
     const int tx = cld_i(items, e), r0 = cld_i(items, e + 1), r1 = cld_i(items, e + 2), w = cld_i(items, e + 3);
-    const int dg = g - a.g_base;
+    const int dg = a.g_stride == 1 ? g - a.g_base : (g - a.g_base) / a.g_stride;
     const int n0 = a.n_base + dg * tp;
     const int src = a.src_base ^ (dg & 1);
     if (r0 < r1) {

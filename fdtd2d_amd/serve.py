@@ -201,7 +201,8 @@ class Server:
                       "host_s": {k: round(v, 3) for k, v in host.items()},
                       "pool_host_s": {k: round(pool.info(k), 3) for k in ("add_s", "rebuild_s", "capture_s", "blocked_s")},
                       "pool_counts": {k: int(pool.info(k)) for k in ("launches", "graph_replays", "eager_launches",
-                                                                      "tail_launches", "rebuilds", "captures")}}
+                                                                      "tail_launches", "rebuilds", "list_switches",
+                                                                      "captures")}}
 
 
 def latency_summary(done, classes=None):

@@ -418,14 +418,19 @@ int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *cou
  * "wg_per_cu_cpml" (persistent work-groups per CU of the interior or fused
  * kernel / of the CPML-class kernel, default 3 / 2), "graph_chunk" (0..16,
  * 0 = no graphs), "list_buffers" (2..4, default 4: device copies of the
- * tile lists; a rebuild of the lists waits for the launches that read the
- * copy it reuses),
+ * tile lists, each kept for the shot set it holds; a rebuild of the lists
+ * waits for the launches that read the copy it reuses), "slice_cells" and
+ * "slice_every" (time slicing, default off: shots of at least slice_cells
+ * cells run only in every slice_every-th launch, so that small shots
+ * finish without waiting for a large shot's full launches; graphs are not
+ * used while such a shot runs),
  * "budget_bytes" (before any shot: carve shots from one arena of that size
  * and refuse shots that do not fit, FDTD2D_ENOMEM).
  * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",
  * "shots", "running", "launches", "cell_updates", "graph_replays",
  * "eager_launches", "tail_launches", "cus", and host-side counters:
  * "rebuilds" / "captures" (main tile lists rebuilt, graphs captured),
+ * "list_switches" (launches that took a kept list copy instead),
  * "rebuild_s", "capture_s", "add_s" (host seconds spent in them and in
  * fdtd2d_pool_add), "blocked_s" (host seconds waiting for launches still on
  * the device, outside fdtd2d_pool_wait and fdtd2d_pool_sync). */

@@ -103,6 +103,73 @@ def test_pool_reshape_midrun_and_late_admission(gpu_device):
             fp64_check(p, g["det"], f"reshaped pool shot {sid}", comp32=g["comp"])
 
 
+def test_pool_time_slicing(gpu_device):
+    """Shots of at least slice_cells cells run only in every slice_every-th
+    launch (their descriptors count turns, tails included) while the small
+    shots run in every launch; the two alternating list sets come from the
+    ring of list copies without rebuilds. Shots are admitted and removed
+    (slots reused) while slicing, the period changes mid-run and slicing is
+    switched off; every shot stays equal to the oracle."""
+    small = _problems(6161, 6, table=False, sizes=(80, 140), steps=(40, 75))
+    large = _problems(6162, 3, table=False, sizes=(300, 331), steps=(45, 70))
+    quick = _problems(6163, 1, table=False, sizes=(80, 140), nsteps=21)[0]  # done in 3 launches
+    probs = [quick, large[0], small[1], small[2], large[1]]
+    with F.Pool(gpu_device, tblock=8, graph_chunk=4, slice_cells=250 * 250, slice_every=3) as pool:
+        ids = [pool.add(p) for p in probs]
+        pool.step(5)
+        print({k: pool.info(k) for k in ("launches", "rebuilds", "list_switches", "graph_replays")}, flush=True)
+        assert pool.info("list_switches") > 0 and pool.info("graph_replays") == 0
+        # a finished small shot leaves (its slot comes back), two join
+        pool.sync()
+        done = [sid for sid, p in zip(ids, probs) if pool.state(sid)[2]]
+        assert ids[0] in done
+        results = {}
+        for sid in done:
+            results[sid] = (_pool_state(pool, sid), probs[ids.index(sid)])
+            pool.remove(sid)
+        for p in (small[3], large[2]):
+            probs.append(p)
+            ids.append(pool.add(p))
+        pool.step(4)
+        pool.set_option("slice_every", 2)
+        ids.append(pool.add(small[4]))
+        probs.append(small[4])
+        pool.step(5)
+        pool.set_option("slice_cells", 0)
+        ids.append(pool.add(small[5]))
+        probs.append(small[5])
+        _drain(pool, 3)
+        print({k: pool.info(k) for k in ("launches", "rebuilds", "list_switches", "tail_launches")}, flush=True)
+        for sid, p in zip(ids, probs):
+            if sid in results and results[sid][1] is p:
+                g = results[sid][0]
+            else:
+                g = _pool_state(pool, sid)
+            _same(g, _oracle_state(p), f"sliced pool shot {sid} {p.nx}x{p.ny} steps {p.nsteps}")
+            fp64_check(p, g["det"], f"sliced pool shot {sid}", comp32=g["comp"])
+
+
+def test_pool_time_slicing_large_shots_alone(gpu_device):
+    """Once only sliced shots are left, launches between their turns are
+    skipped (the launch counter moves on to the next turn, on the host and
+    the device) instead of stalling the pool."""
+    small = _problems(7171, 1, table=False, sizes=(80, 100), nsteps=8)[0]  # one launch
+    large = _problems(7172, 2, table=False, sizes=(300, 331), steps=(40, 60))
+    with F.Pool(gpu_device, tblock=8, graph_chunk=0, slice_cells=250 * 250, slice_every=5) as pool:
+        ids = [pool.add(p) for p in [small] + large]
+        pool.step(2)  # launch 0: every shot; launch 1: the small one has finished, the large ones wait
+        pool.sync()
+        assert pool.state(ids[0])[2]
+        g0 = pool.info("launches")
+        pool.step(3)
+        assert pool.info("launches") > g0 + 3  # the skipped launches count
+        _drain(pool, 3)
+        print({k: pool.info(k) for k in ("launches", "eager_launches", "tail_launches")}, flush=True)
+        for sid, p in zip(ids, [small] + large):
+            g = _pool_state(pool, sid)
+            _same(g, _oracle_state(p), f"large-only sliced shot {sid} {p.nx}x{p.ny}")
+
+
 def test_pool_fused_toggled_midrun_table_medium(gpu_device):
     """Switching between one kernel per tile class and the fused kernel (whose
     table-medium form leaves corner and general pieces to a second kernel)
