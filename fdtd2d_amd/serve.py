@@ -12,6 +12,9 @@ the queue does not fit, later shots that do are admitted around it
 boundary, so their work overlaps on the device with everything already
 running; finished shots are read back and removed, freeing their bytes for
 the queue. A request completes when the results of all its shots are read.
+With the pool's time-slicing options (slice_cells, slice_every) large shots
+run only in every slice_every-th launch, so a small shot admitted beside
+them is not held to a large shot's launch time at every one of its steps.
 
 Latency = completion - arrival, reported as p50/p99 over the completed
 requests (per size class where the stream has classes), next to completed
