@@ -429,7 +429,8 @@ def bench_serve(args) -> int:
     # time slicing: the 16384^2 shots of the mixed stream sit out launches so
     # that the 1024^2 shots do not wait for a full large launch every step
     every = args.slice_every if args.slice_every >= 0 else (8 if args.mode == "serve-mixed" else 1)
-    slicing = {"slice_every": every, "slice_cells": args.slice_cells if every > 1 else 0}
+    slicing = {"slice_every": every, "slice_cells": args.slice_cells if every > 1 else 0,
+               "band_small": args.band_small if every > 1 else 0}
     for name, value in slicing.items():
         srv.pool.set_option(name, value)
     done, st = srv.run(reqs, max_seconds=args.duration + args.drain, log=log)
@@ -492,6 +493,8 @@ def main() -> int:
                     help="serving: shots of at least --slice-cells cells run in every Nth launch only "
                          "(1 = off; default 8 for serve-mixed, off for serve-sweep)")
     ap.add_argument("--slice-cells", type=float, default=16e6)
+    ap.add_argument("--band-small", type=int, default=64,
+                    help="serving with time slicing: band rows of the shots that are not sliced")
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--plan-shape", choices=["cache", "tune", "defaults"], default="cache",
