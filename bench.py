@@ -392,6 +392,17 @@ def bench_batch(args) -> int:
     return 0 if same is not False and not bad else 1
 
 
+def slicing_options(args) -> dict:
+    """Pool time-slicing options of a serving run: the 16384^2 shots of the
+    mixed stream sit out launches (every 8th by default) so that the 1024^2
+    shots do not wait for a full large launch at every step; the sweep
+    stream, whose shots are alike, runs unsliced unless asked."""
+    every = args.slice_every if args.slice_every >= 0 else (8 if args.mode == "serve-mixed" else 1)
+    on = every > 1
+    return {"slice_every": every, "slice_cells": args.slice_cells if on else 0,
+            "band_small": args.band_small if on else 0}
+
+
 def bench_serve(args) -> int:
     """Served streams through the continuous-batching loop: frequency sweeps
     (--mode serve-sweep) or the mixed 1024^2 / 16384^2 stream (--mode
@@ -426,11 +437,7 @@ def bench_serve(args) -> int:
     for name, value in shape.items():
         srv.pool.set_option(name, value)
     log(f"pool shape {shape} ({source}; stream mix {key})")
-    # time slicing: the 16384^2 shots of the mixed stream sit out launches so
-    # that the 1024^2 shots do not wait for a full large launch every step
-    every = args.slice_every if args.slice_every >= 0 else (8 if args.mode == "serve-mixed" else 1)
-    slicing = {"slice_every": every, "slice_cells": args.slice_cells if every > 1 else 0,
-               "band_small": args.band_small if every > 1 else 0}
+    slicing = slicing_options(args)
     for name, value in slicing.items():
         srv.pool.set_option(name, value)
     done, st = srv.run(reqs, max_seconds=args.duration + args.drain, log=log)

@@ -131,3 +131,16 @@ def test_pmc_total_sums_every_dispatch(tmp_path, capsys):
     assert "pool_kernel<8, 1, 1, 1, 0>" in out and "pool_set" in out and "copyBuffer" not in out
     total = (2 * 1e6 * 1024 * 2 + 2 * 5e5 * 1024) / 1e9
     assert f"total: {total:.2f} GB in 2.0 ms of dispatches" in out
+
+
+def test_serving_time_slicing_defaults():
+    from types import SimpleNamespace as NS
+
+    mixed = bench.slicing_options(NS(mode="serve-mixed", slice_every=-1, slice_cells=16e6, band_small=64))
+    assert mixed == {"slice_every": 8, "slice_cells": 16e6, "band_small": 64}
+    sweep = bench.slicing_options(NS(mode="serve-sweep", slice_every=-1, slice_cells=16e6, band_small=64))
+    assert sweep == {"slice_every": 1, "slice_cells": 0, "band_small": 0}
+    off = bench.slicing_options(NS(mode="serve-mixed", slice_every=1, slice_cells=16e6, band_small=64))
+    assert off["slice_cells"] == 0 and off["band_small"] == 0
+    forced = bench.slicing_options(NS(mode="serve-sweep", slice_every=4, slice_cells=1e6, band_small=0))
+    assert forced == {"slice_every": 4, "slice_cells": 1e6, "band_small": 0}
