@@ -361,6 +361,20 @@ def bench_batch(args) -> int:
         cpu = {"value": round(rate / (cells / len(setups)), 6), "unit": "sims/s", "cores": 1, "kind": "port",
                "sample": f"FP32 CPU oracle, shots 0-{nshot - 1} of the mix x {nstep} steps ({ssum:.1f} s, single "
                          f"thread, {rate / 1e9:.3f} Gcell/s), scaled to whole shots"}
+        # the batched detector traces of the smallest shots, whole runs,
+        # against the FP64 oracle (all host cores)
+        small = sorted(range(len(setups)), key=lambda k: probs[k].nx * probs[k].ny)[: args.fp64_shots]
+        if small:
+            c0 = time.perf_counter()
+            worst = 0.0
+            for k in small:
+                ref = Oracle(setups[k], precision=64, threads=max(1, host_cpu()["nproc"]))
+                ref.run(probs[k].nsteps)
+                worst = max(worst, rel_l2(det_b[k], ref.read_detectors()[0]))
+                ref.close()
+            cpu["fp64_check"] = {"shots": small, "steps": "all", "worst_det_rel_l2": float(f"{worst:.3e}"),
+                                 "bound": 1e-5, "seconds": round(time.perf_counter() - c0, 1)}
+            log(f"FP64 oracle, shots {small}: worst batched detector rel L2 {worst:.3e}")
     line = {
         "metric": "batched sims/s",
         "value": round(len(setups) / tb, 3),
@@ -390,6 +404,17 @@ def bench_batch(args) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0 if same is not False and not bad else 1
+
+
+def rel_l2(a, b) -> float:
+    """Relative L2 error of a against b, in float64 (0 when both are 0)."""
+    import numpy as np
+
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = float(np.sqrt((b * b).sum()))
+    num = float(np.sqrt(((a - b) ** 2).sum()))
+    return num / den if den > 0 else (0.0 if num == 0 else float("inf"))
 
 
 def slicing_options(args) -> dict:
@@ -490,6 +515,10 @@ def main() -> int:
     ap.add_argument("--tblock", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=48)
+    ap.add_argument("--fp64-steps", type=int, default=512,
+                    help="single grid, CPU leg: steps of the FP64 oracle the GPU traces are checked against")
+    ap.add_argument("--fp64-shots", type=int, default=4,
+                    help="batch, CPU leg: smallest shots checked whole against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_8192.json"))
     ap.add_argument("--mode", choices=["single", "batch", "serve-sweep", "serve-mixed"], default="single")
@@ -634,6 +663,19 @@ def main() -> int:
         ref.close()
         cpu["all_cores"] = {"value": round(cells * args.cpu_steps / cs_all / 1e9, 4), "cores": nthr}
         log(f"CPU oracle: {cpu['value']} Gcell/s single-threaded, {cpu['all_cores']['value']} on {nthr} threads")
+        # the GPU's detector traces against the FP64 oracle of the same grid
+        # over its first --fp64-steps steps (all host cores)
+        k = min(args.fp64_steps, total)
+        if k > 0:
+            ref = Oracle(setup, precision=64, threads=nthr)
+            c0 = time.perf_counter()
+            ref.run(k)
+            det64 = ref.read_detectors()[0][: k + 1]
+            ref.close()
+            err = rel_l2(det[: k + 1], det64)
+            cpu["fp64_check"] = {"steps": k, "det_rel_l2": float(f"{err:.3e}"), "bound": 1e-5,
+                                 "seconds": round(time.perf_counter() - c0, 1)}
+            log(f"FP64 oracle, first {k} steps: GPU detector traces rel L2 {err:.3e}")
 
     line = {
         "metric": "Gcell-updates/s",

@@ -144,3 +144,13 @@ def test_serving_time_slicing_defaults():
     assert off["slice_cells"] == 0 and off["band_small"] == 0
     forced = bench.slicing_options(NS(mode="serve-sweep", slice_every=4, slice_cells=1e6, band_small=0))
     assert forced == {"slice_every": 4, "slice_cells": 1e6, "band_small": 0}
+
+
+def test_rel_l2():
+    import numpy as np
+
+    b = np.array([3.0, 4.0], np.float32)
+    assert bench.rel_l2(b, b) == 0.0
+    assert abs(bench.rel_l2(np.array([3.0, 4.5]), b) - 0.1) < 1e-12
+    assert bench.rel_l2(np.zeros(2), np.zeros(2)) == 0.0
+    assert bench.rel_l2(np.ones(2), np.zeros(2)) == float("inf")
