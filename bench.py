@@ -293,6 +293,39 @@ def stats_summary(path: str) -> int:
     return 0
 
 
+def busy_summary(path: str, match: str = "pool_kernel", gap_ms: float = 1.0) -> dict:
+    """From a rocprofv3 kernel trace (kernel_trace.csv): the union of the
+    intervals of the kernels whose name contains `match` over the span from
+    the first to the last of them, the idle share of that span, and the gaps
+    between them longer than `gap_ms`."""
+    import csv
+
+    iv = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if match in r["Kernel_Name"]:
+                iv.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    iv.sort()
+    merged = []
+    for a, b in iv:
+        if merged and a <= merged[-1][1]:
+            merged[-1][1] = max(merged[-1][1], b)
+        else:
+            merged.append([a, b])
+    if not merged:
+        return {"kernels": 0}
+    busy = sum(b - a for a, b in merged)
+    span = merged[-1][1] - merged[0][0]
+    gaps = sorted((merged[k + 1][0] - merged[k][1] for k in range(len(merged) - 1)), reverse=True)
+    long = [g for g in gaps if g > gap_ms * 1e6]
+    out = {"kernels": len(iv), "span_s": span / 1e9, "busy_s": busy / 1e9,
+           "idle_frac": 1 - busy / span if span > 0 else 0.0,
+           "gaps_over": {"ms": gap_ms, "count": len(long), "sum_s": sum(long) / 1e9,
+                         "largest_ms": [round(g / 1e6, 1) for g in long[:5]]}}
+    print(json.dumps(out))
+    return out
+
+
 def bench_batch(args) -> int:
     """The 256-shot uneven mix (seed 3): every shot through one pool (persistent
     kernels, graph chunks, autotuned shape) against looping the plain
@@ -506,6 +539,9 @@ def main() -> int:
     ap.add_argument("--pmc-summary", nargs=4, default=None)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--stats-summary", default=None)
+    ap.add_argument("--busy-summary", default=None, metavar="KERNEL_TRACE_CSV",
+                    help="busy and idle time of the pool kernels in a rocprofv3 kernel trace")
+    ap.add_argument("--busy-kernel", default="pool_kernel")
     ap.add_argument("--sq-summary", nargs="+", default=None)
     ap.add_argument("--sq-kernel", default=None, help="with --sq-summary: every kernel whose name contains this")
     ap.add_argument("--pmc-total", nargs=4, default=None)
@@ -551,6 +587,9 @@ def main() -> int:
         return pmc_summary(args.pmc_summary, args.copy_mib, args.traffic_file)
     if args.stats_summary:
         return stats_summary(args.stats_summary)
+    if args.busy_summary:
+        busy_summary(args.busy_summary, match=args.busy_kernel)
+        return 0
     if args.sq_summary:
         return sq_summary(args.sq_summary, match=args.sq_kernel)
     if args.pmc_total:

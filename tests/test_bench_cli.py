@@ -154,3 +154,24 @@ def test_rel_l2():
     assert abs(bench.rel_l2(np.array([3.0, 4.5]), b) - 0.1) < 1e-12
     assert bench.rel_l2(np.zeros(2), np.zeros(2)) == 0.0
     assert bench.rel_l2(np.ones(2), np.zeros(2)) == float("inf")
+
+
+def test_busy_summary_unions_overlapping_kernels(tmp_path):
+    path = tmp_path / "run_kernel_trace.csv"
+    rows = [  # (name, start ns, end ns): two pool kernels side by side, a copy, a 5 ms gap
+        ("void pool_kernel<8, 1, 1, 1, 0>(...)", 0, 4_000_000),
+        ("void pool_kernel<8, 1, 1, 1, 1>(...)", 1_000_000, 5_000_000),
+        ("__amd_rocclr_copyBuffer", 5_000_000, 9_000_000),
+        ("void pool_kernel<8, 1, 1, 1, 0>(...)", 10_000_000, 12_000_000),
+        ("void pool_kernel<8, 1, 1, 1, 0>(...)", 12_500_000, 13_000_000),
+    ]
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        for n, a, b in rows:
+            w.writerow({"Kernel_Name": n, "Start_Timestamp": a, "End_Timestamp": b})
+    out = bench.busy_summary(str(path))
+    assert out["kernels"] == 4
+    assert abs(out["span_s"] - 0.013) < 1e-12 and abs(out["busy_s"] - 0.0075) < 1e-12
+    assert abs(out["idle_frac"] - 5.5 / 13) < 1e-12
+    assert out["gaps_over"]["count"] == 1 and out["gaps_over"]["largest_ms"] == [5.0]
