@@ -452,13 +452,14 @@ def rel_l2(a, b) -> float:
 
 def slicing_options(args) -> dict:
     """Pool time-slicing options of a serving run: the 16384^2 shots of the
-    mixed stream sit out launches (every 8th by default) so that the 1024^2
-    shots do not wait for a full large launch at every step; the sweep
-    stream, whose shots are alike, runs unsliced unless asked."""
-    every = args.slice_every if args.slice_every >= 0 else (8 if args.mode == "serve-mixed" else 1)
+    mixed stream run only in every 16th launch by default, so that the
+    1024^2 shots do not wait for a full large launch at every step, and the
+    small shots take shorter bands meanwhile; the sweep stream, whose shots
+    are alike, runs unsliced unless asked."""
+    every = args.slice_every if args.slice_every >= 0 else (16 if args.mode == "serve-mixed" else 1)
     on = every > 1
     return {"slice_every": every, "slice_cells": args.slice_cells if on else 0,
-            "band_small": args.band_small if on else 0}
+            "band_small": args.band_small if on else 0, "band_cpml_small": args.band_cpml_small if on else 0}
 
 
 def bench_serve(args) -> int:
@@ -563,10 +564,12 @@ def main() -> int:
     ap.add_argument("--drain", type=float, default=120.0)
     ap.add_argument("--slice-every", type=int, default=-1,
                     help="serving: shots of at least --slice-cells cells run in every Nth launch only "
-                         "(1 = off; default 8 for serve-mixed, off for serve-sweep)")
+                         "(1 = off; default 16 for serve-mixed, off for serve-sweep)")
     ap.add_argument("--slice-cells", type=float, default=16e6)
     ap.add_argument("--band-small", type=int, default=64,
                     help="serving with time slicing: band rows of the shots that are not sliced")
+    ap.add_argument("--band-cpml-small", type=int, default=32,
+                    help="serving with time slicing: CPML sub-band rows of the shots that are not sliced")
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--plan-shape", choices=["cache", "tune", "defaults"], default="cache",

@@ -193,8 +193,9 @@ struct fdtd2d_pool {
   // with g % slice_every == 0 (0 cells: off)
   double slice_cells = 0;
   int slice_every = 1;
-  // band rows of the shots that are not sliced while slicing is on (0: band)
-  int band_small = 0;
+  // band and CPML sub-band rows of the shots that are not sliced while
+  // slicing is on (0: band, band_cpml)
+  int band_small = 0, band_cpml_small = 0;
   // host seconds: rebuilding and uploading main lists, capturing graphs,
   // blocked on launches still running (outside fdtd2d_pool_wait/_sync),
   // and adding shots
@@ -306,11 +307,16 @@ bool sliced(const fdtd2d_pool *pp, const fdtd2d_plan *pl) {
   return sliced_cells(pp, static_cast<double>(pl->L.nx) * pl->L.ny);
 }
 
-// Band rows of a shot of `cells` cells: shorter bands for the shots that run
-// in every launch while large ones are sliced (a launch carrying only them
-// lasts about one band's row loop).
+// Band (and CPML sub-band) rows of a shot of `cells` cells: shorter bands
+// for the shots that run in every launch while large ones are sliced (a
+// launch carrying only them lasts about one band's row loop).
 int shot_band(const fdtd2d_pool *pp, double cells) {
   return slicing(pp) && pp->band_small > 0 && !sliced_cells(pp, cells) ? pp->band_small : pp->band;
+}
+
+int shot_band_cpml(const fdtd2d_pool *pp, double cells) {
+  return slicing(pp) && pp->band_cpml_small > 0 && !sliced_cells(pp, cells) ? pp->band_cpml_small
+                                                                              : pp->band_cpml;
 }
 
 // Fills the host copy of shot `slot`'s descriptor: its state at the next
@@ -490,8 +496,9 @@ int reshape(fdtd2d_pool *pp) {
     if (!sh.live) continue;
     sh.pl->T = pp->T;
     sh.pl->C = pp->C;
-    sh.pl->band = shot_band(pp, static_cast<double>(sh.pl->L.nx) * sh.pl->L.ny);
-    sh.pl->band_cpml = pp->band_cpml;
+    const double cells = static_cast<double>(sh.pl->L.nx) * sh.pl->L.ny;
+    sh.pl->band = shot_band(pp, cells);
+    sh.pl->band_cpml = shot_band_cpml(pp, cells);
     int rc = set_tiling(sh.pl);
     if (rc) return rc;
     fill_desc(pp, static_cast<int>(s));
@@ -675,9 +682,9 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
     if (v < 0) return set_error(FDTD2D_EINVAL, "band must be >= 0");
     pp->band = static_cast<int>(v);
     shape = true;
-  } else if (n == "band_small") {
-    if (v < 0) return set_error(FDTD2D_EINVAL, "band_small must be >= 0");
-    pp->band_small = static_cast<int>(v);
+  } else if (n == "band_small" || n == "band_cpml_small") {
+    if (v < 0) return set_error(FDTD2D_EINVAL, n + " must be >= 0");
+    (n == "band_small" ? pp->band_small : pp->band_cpml_small) = static_cast<int>(v);
     shape = true;
   } else if (n == "band_cpml") {
     if (v < 0) return set_error(FDTD2D_EINVAL, "band_cpml must be >= 0");
@@ -720,7 +727,7 @@ int fdtd2d_pool_set_option(fdtd2d_pool *pp, const char *name, double value) {
     retire(pp, pp->g);
     if (n == "slice_cells") pp->slice_cells = v;
     else pp->slice_every = static_cast<int>(v);
-    if (pp->band_small > 0) return reshape(pp);  // a shot's band follows its class
+    if (pp->band_small > 0 || pp->band_cpml_small > 0) return reshape(pp);  // a shot's bands follow its class
     for (size_t s = 0; s < pp->shots.size(); ++s)
       if (pp->shots[s].live) fill_desc(pp, static_cast<int>(s));
     return copy_descs(pp, 0, pp->shots.size());
@@ -746,6 +753,7 @@ int fdtd2d_pool_get_info(const fdtd2d_pool *pp, const char *name, double *value)
   else if (n == "band") *value = pp->band;
   else if (n == "band_cpml") *value = pp->band_cpml;
   else if (n == "band_small") *value = pp->band_small;
+  else if (n == "band_cpml_small") *value = pp->band_cpml_small;
   else if (n == "prefetch") *value = pp->D;
   else if (n == "cols_per_lane") *value = pp->C;
   else if (n == "wg_per_cu") *value = pp->wg_per_cu;
@@ -795,7 +803,8 @@ int fdtd2d_pool_add(fdtd2d_pool *pp, const fdtd2d_problem *p, int *shot) {
   }
   fdtd2d_plan *pl = nullptr;
   rc = create_plan(p, pp->device, true, pp->stream, pp->T, pp->C,
-                   shot_band(pp, static_cast<double>(p->nx) * p->ny), pp->band_cpml, base, &pl);
+                   shot_band(pp, static_cast<double>(p->nx) * p->ny),
+                   shot_band_cpml(pp, static_cast<double>(p->nx) * p->ny), base, &pl);
   if (rc) {
     if (base) pp->arena.give(base);
     return rc;

@@ -423,9 +423,10 @@ int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *cou
  * "slice_every" (time slicing, default off: shots of at least slice_cells
  * cells run only in every slice_every-th launch, so that small shots
  * finish without waiting for a large shot's full launches; graphs are not
- * used while such a shot runs), "band_small" (band rows of the other shots
- * while slicing is on, 0 = "band": a launch carrying only small shots lasts
- * about one band's row loop),
+ * used while such a shot runs), "band_small" / "band_cpml_small" (band and
+ * CPML sub-band rows of the other shots while slicing is on, 0 = "band" /
+ * "band_cpml": a launch carrying only small shots lasts about one band's
+ * row loop),
  * "budget_bytes" (before any shot: carve shots from one arena of that size
  * and refuse shots that do not fit, FDTD2D_ENOMEM).
  * Info (fdtd2d_pool_get_info): the options, "bytes_in_use", "largest_free",

@@ -136,14 +136,16 @@ def test_pmc_total_sums_every_dispatch(tmp_path, capsys):
 def test_serving_time_slicing_defaults():
     from types import SimpleNamespace as NS
 
-    mixed = bench.slicing_options(NS(mode="serve-mixed", slice_every=-1, slice_cells=16e6, band_small=64))
-    assert mixed == {"slice_every": 8, "slice_cells": 16e6, "band_small": 64}
-    sweep = bench.slicing_options(NS(mode="serve-sweep", slice_every=-1, slice_cells=16e6, band_small=64))
-    assert sweep == {"slice_every": 1, "slice_cells": 0, "band_small": 0}
-    off = bench.slicing_options(NS(mode="serve-mixed", slice_every=1, slice_cells=16e6, band_small=64))
-    assert off["slice_cells"] == 0 and off["band_small"] == 0
-    forced = bench.slicing_options(NS(mode="serve-sweep", slice_every=4, slice_cells=1e6, band_small=0))
-    assert forced == {"slice_every": 4, "slice_cells": 1e6, "band_small": 0}
+    kw = dict(slice_cells=16e6, band_small=64, band_cpml_small=32)
+    mixed = bench.slicing_options(NS(mode="serve-mixed", slice_every=-1, **kw))
+    assert mixed == {"slice_every": 16, "slice_cells": 16e6, "band_small": 64, "band_cpml_small": 32}
+    sweep = bench.slicing_options(NS(mode="serve-sweep", slice_every=-1, **kw))
+    assert sweep == {"slice_every": 1, "slice_cells": 0, "band_small": 0, "band_cpml_small": 0}
+    off = bench.slicing_options(NS(mode="serve-mixed", slice_every=1, **kw))
+    assert off["slice_cells"] == 0 and off["band_small"] == 0 and off["band_cpml_small"] == 0
+    forced = bench.slicing_options(NS(mode="serve-sweep", slice_every=4, slice_cells=1e6, band_small=0,
+                                      band_cpml_small=0))
+    assert forced == {"slice_every": 4, "slice_cells": 1e6, "band_small": 0, "band_cpml_small": 0}
 
 
 def test_rel_l2():

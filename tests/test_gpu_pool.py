@@ -109,13 +109,14 @@ def test_pool_time_slicing(gpu_device):
     shots run in every launch; the two alternating list sets come from the
     ring of list copies without rebuilds. Shots are admitted and removed
     (slots reused) while slicing, the period changes mid-run and slicing is
-    switched off (the small shots' 32-row bands follow, retiled); every shot
-    stays equal to the oracle."""
+    switched off (the small shots' 32-row bands and CPML sub-bands follow,
+    retiled); every shot stays equal to the oracle."""
     small = _problems(6161, 6, table=False, sizes=(80, 140), steps=(40, 75))
     large = _problems(6162, 3, table=False, sizes=(300, 331), steps=(45, 70))
     quick = _problems(6163, 1, table=False, sizes=(80, 140), nsteps=21)[0]  # done in 3 launches
     probs = [quick, large[0], small[1], small[2], large[1]]
-    with F.Pool(gpu_device, tblock=8, graph_chunk=4, slice_cells=250 * 250, slice_every=3, band_small=32) as pool:
+    with F.Pool(gpu_device, tblock=8, graph_chunk=4, slice_cells=250 * 250, slice_every=3, band_small=32,
+                band_cpml_small=32) as pool:
         ids = [pool.add(p) for p in probs]
         pool.step(5)
         print({k: pool.info(k) for k in ("launches", "rebuilds", "list_switches", "graph_replays")}, flush=True)
