@@ -403,10 +403,14 @@ int fdtd2d_plan_kernel_time(fdtd2d_plan *plan, int cls, double *avg_ms, int *cou
  * (F/FDTD_cuda.cu:384-441). Results of every shot are bitwise those of a
  * single plan of the same problem.
  *
- * A pool launch advances every unfinished shot by min(tblock, steps left);
+ * A pool launch advances every unfinished shot by min(tblock, steps left)
+ * (with time slicing, a sliced shot only at its turns);
  * fdtd2d_pool_step queues `nlaunch` of them (asynchronously; chunks of
  * `graph_chunk` launches are replayed from a captured hipGraph when no shot
- * ends inside them). Shots can be added and removed between steps.
+ * ends inside them). When only sliced shots run, the launches between their
+ * turns are skipped: they launch nothing but count in "launches" and in
+ * fdtd2d_pool_wait's max_pending. Shots can be added and removed between
+ * steps.
  *
  * Options (fdtd2d_pool_set_option): "tblock" (1..8, default 8), "band"
  * (default 128, 0 = per shot), "band_cpml" (default 64, 0 = per shot),
