@@ -31,6 +31,7 @@
 #include "errors.h"
 #include "plan_impl.h"
 #include "plan_internal.h"
+#include "pool_policy.h"
 #include "validate.h"
 
 using fdtd2d::set_error;
@@ -270,14 +271,13 @@ int rebuild_main(fdtd2d_pool *pp, const std::vector<int> &slots) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<int4> lists[fdtd2d::kPoolLists];
   for (int s : slots) append_shot(pp, s, lists);
-  // an unused buffer, else the least recently read one other than the current
-  int nb = -1;
+  bool valid[fdtd2d_pool::kListBufs];
+  int last_g[fdtd2d_pool::kListBufs];
   for (int i = 0; i < pp->nbufs; ++i) {
-    if (i == pp->cur && pp->nbufs > 1) continue;
-    if (nb < 0 || (!pp->bufs[i].valid && pp->bufs[nb].valid) ||
-        (pp->bufs[i].valid == pp->bufs[nb].valid && pp->bufs[i].last_g < pp->bufs[nb].last_g))
-      nb = i;
+    valid[i] = pp->bufs[i].valid;
+    last_g[i] = pp->bufs[i].last_g;
   }
+  const int nb = fdtd2d::policy::pick_copy(valid, last_g, pp->nbufs, pp->cur);
   fdtd2d_pool::ListBuf &b = pp->bufs[nb];
   const auto tw = std::chrono::steady_clock::now();
   retire(pp, b.last_g + 1);  // launches that read this buffer have finished
@@ -328,7 +328,7 @@ void fill_desc(fdtd2d_pool *pp, int slot) {
   // a sliced shot runs in every slice_every-th launch from the next one on
   const int k = sliced(pp, pl) ? pp->slice_every : 1;
   a.g_stride = k;
-  a.g_base = (pp->g + k - 1) / k * k;
+  a.g_base = fdtd2d::policy::first_turn(pp->g, k);
   a.n_base = pl->n;
   a.src_base = pl->src;
 }
@@ -537,7 +537,7 @@ int step_impl(fdtd2d_pool *pp, int nlaunch) {
       if (!any_sliced) break;
       // only sliced shots run and this is not their turn: move the launch
       // counter (host and device) on to their next turn, launching nothing
-      const int skip = pp->slice_every - pp->g % pp->slice_every;
+      const int skip = fdtd2d::policy::to_next_turn(pp->g, pp->slice_every);
       hipError_t e = fdtd2d::launch_pool_tick(pp->d_q, 0, skip, pp->stream);
       if (e != hipSuccess) return hip_fail(e, "pool tick launch");
       pp->g += skip;
