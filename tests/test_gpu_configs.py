@@ -13,7 +13,9 @@ detector error against the FP64 oracle (asserted within 1e-5):
 * a 16384^2 shot of the mixed serving stream (CPML 16, 32 elements), as a
   plan at the default shape and inside a pool beside two 1024^2 shots;
 * the first 16 shots of the 256-shot batch mix (256-2048 cells per side)
-  through one pool: a captured 8-launch graph chunk plus a tail launch.
+  through one pool: a captured 8-launch graph chunk plus a tail launch;
+* three frequencies of one served sweep request (640^2, CPML, the sweep's
+  plasma) for their full 3000 steps through one pool at the serving shape.
 """
 
 from __future__ import annotations
@@ -204,3 +206,31 @@ def test_batch_mix_first_16_shots(gpu_device):
             det, comp = _same_traces(shot, ref, f"mix shot {sid}")
             ref.close()
             fp64_check(s, det, f"mix shot {sid} {sizes[sid]}", threads=THREADS, comp32=comp)
+
+
+def test_sweep_request_full_length(gpu_device):
+    """Three frequencies of served sweep request 12 (seed 4: a 640^2 grid, one
+    plasma profile, 3000 steps each) through one pool at the serving launch
+    shape (the batch mix's cached shape), with graph chunks: every field and
+    trace bitwise equal to the FP32 oracle at the end of the full run."""
+    shots = [F.Setup.workload(4, seed=4, shot=12 * 64 + k, steps=0) for k in range(3)]
+    sizes = {(int(s.get("Imax")), int(s.get("Jmax")), int(s.get("nMax"))) for s in shots}
+    print(f"sweep request 12: {sizes}", flush=True)
+    assert sizes == {(640, 640, 3000)}
+    shape = {"fused": 0, "tblock": 8, "cols_per_lane": 1, "band": 128, "wg_per_cu": 3, "band_cpml": 64,
+             "wg_per_cu_cpml": 2, "prefetch": 1}
+    with F.Pool(gpu_device, graph_chunk=8, **shape) as pool:
+        ids = [pool.add(s) for s in shots]
+        while pool.info("running") > 0:
+            pool.step(8)
+        pool.sync()
+        print({k: pool.info(k) for k in ("launches", "graph_replays", "eager_launches")}, flush=True)
+        assert pool.info("graph_replays") > 0
+        for sid, s in zip(ids, shots):
+            ref = Oracle(s, 32, threads=THREADS)
+            ref.run(3000)
+            shot = _PoolShot(pool, sid)
+            compare_fields_lean(shot, ref, f"sweep shot {sid}", psi=False)
+            det, comp = _same_traces(shot, ref, f"sweep shot {sid}")
+            ref.close()
+            fp64_check(s, det, f"sweep shot {sid} 640x640 x 3000 steps", threads=THREADS, comp32=comp)
