@@ -25,6 +25,7 @@
 #include "errors.h"
 #include "plan_impl.h"
 #include "plan_internal.h"
+#include "runs.h"
 #include "validate.h"
 
 using fdtd2d::set_error;
@@ -607,32 +608,6 @@ int upload(fdtd2d_plan *pl, size_t off, const void *src, size_t bytes) {
   return FDTD2D_OK;
 }
 
-// Appends the maximal runs [first, end) of cells with Ne > 0 in one row of n
-// cells; positive(i) says whether cell i has Ne > 0. Blocks of 64 cells with
-// no change of state are passed over after one counting pass (the plasma of
-// a row is one or a few long runs).
-template <class Positive>
-static void append_runs(int n, Positive positive, std::vector<int2> &runs) {
-  constexpr int kBlock = 64;
-  int run = -1;  // first column of the open run
-  for (int i = 0; i < n; i += kBlock) {
-    const int m = std::min(kBlock, n - i);
-    int count = 0;
-    for (int k = 0; k < m; ++k) count += positive(i + k);
-    if (count == (run >= 0 ? m : 0)) continue;
-    for (int k = 0; k < m; ++k) {
-      const bool pos = positive(i + k);
-      if (pos && run < 0) {
-        run = i + k;
-      } else if (!pos && run >= 0) {
-        runs.push_back(make_int2(run, i + k));
-        run = -1;
-      }
-    }
-  }
-  if (run >= 0) runs.push_back(make_int2(run, n));
-}
-
 // Per row of the problem: the runs of Ne > 0 (ne_runs, indexed by
 // ne_run_off) and their column span (ne_lo / ne_hi: first and last plasma
 // column, INT32_MAX / -1 for a row without plasma).
@@ -647,16 +622,17 @@ static void plasma_runs(const fdtd2d_problem *p, fdtd2d_plan *pl) {
     mat_pos.resize(p->nmaterials);
     for (int m = 0; m < p->nmaterials; ++m) mat_pos[m] = p->ne[m] > 0.f;
   }
+  auto emit = [pl](int a, int b) { pl->ne_runs.push_back(make_int2(a, b)); };
   for (int j = 0; j < ny; ++j) {
     const size_t row = static_cast<size_t>(j) * nx;
     const size_t first = pl->ne_runs.size();
     if (p->medium_kind == FDTD2D_MEDIUM_TABLE) {
       const int32_t *mat = p->material + row;
       const uint8_t *pos = mat_pos.data();
-      append_runs(nx, [=](int i) { return static_cast<int>(pos[mat[i]]); }, pl->ne_runs);
+      scan_runs(nx, [=](int i) { return static_cast<int>(pos[mat[i]]); }, emit);
     } else {
       const float *ne = p->density + row;
-      append_runs(nx, [=](int i) { return static_cast<int>(ne[i] > 0.f); }, pl->ne_runs);
+      scan_runs(nx, [=](int i) { return static_cast<int>(ne[i] > 0.f); }, emit);
     }
     if (pl->ne_runs.size() > first) {
       pl->ne_lo[j] = pl->ne_runs[first].x;

@@ -1,7 +1,9 @@
-"""Host-side scheduling arithmetic of pools (fdtd2d_amd/csrc/pool_policy.h),
-compiled alone with g++: the turns of time-sliced shots and the list copy a
-rebuild writes. The GPU pool tests check the same rules end to end (results
-equal to the oracle); these pin the arithmetic itself on the CPU."""
+"""Host-side pieces of plan and pool set-up, compiled alone with g++: the
+scheduling arithmetic of pools (fdtd2d_amd/csrc/pool_policy.h: the turns of
+time-sliced shots, the list copy a rebuild writes) and the block scan for
+plasma runs the tiler uses (runs.h), against brute force. The GPU tests
+check the same rules end to end (results equal to the oracle); these pin
+them on the CPU."""
 
 from __future__ import annotations
 
@@ -59,3 +61,52 @@ def test_pool_policy(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout
     assert r.stdout.strip().endswith("bad 0")
+
+
+RUNS_HARNESS = r"""
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+#include "runs.h"
+int main() {
+  std::mt19937 rng(7);
+  const float vals[] = {0.f, -0.f, 1e18f, -3.f, NAN, 1e-30f, INFINITY};
+  long bad = 0;
+  for (int t = 0; t < 100000; ++t) {
+    const int n = 1 + rng() % 300;
+    std::vector<float> r(n);
+    if (rng() % 3 == 0) {
+      for (auto &v : r) v = vals[rng() % 7];
+    } else {  // one long run, maybe with a stray cell
+      const int a = rng() % n, b = rng() % (n + 1);
+      for (int i = 0; i < n; ++i) r[i] = (i >= a && i < b) ? 1e18f : 0.f;
+      if (rng() % 2) r[rng() % n] = vals[rng() % 7];
+    }
+    std::vector<int> want, got;
+    int run = -1;  // brute force, cell by cell
+    for (int i = 0; i < n; ++i) {
+      if (r[i] > 0.f) { if (run < 0) run = i; }
+      else if (run >= 0) { want.push_back(run); want.push_back(i); run = -1; }
+    }
+    if (run >= 0) { want.push_back(run); want.push_back(n); }
+    const float *ne = r.data();
+    fdtd2d::scan_runs(n, [=](int i) { return static_cast<int>(ne[i] > 0.f); },
+                      [&](int a, int b) { got.push_back(a); got.push_back(b); });
+    bad += got != want;
+  }
+  printf("bad %ld\n", bad);
+  return bad != 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_plasma_run_scan(tmp_path):
+    src = tmp_path / "runs.cpp"
+    src.write_text(RUNS_HARNESS)
+    exe = tmp_path / "runs"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", str(ROOT / "fdtd2d_amd" / "csrc"),
+                    str(src), "-o", str(exe)], check=True, capture_output=True, text=True, timeout=120)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "bad 0", r.stdout
